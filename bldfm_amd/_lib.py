@@ -1,0 +1,211 @@
+"""ctypes binding of the BLDFM HIP library (``include/bldfm_amd.h``).
+
+The library is loaded once per process, after torch (so it binds to the HIP
+runtime torch loaded).  There is no CPU fallback: if the library is missing
+or no HIP device is visible, every entry point raises.
+"""
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "_bldfm_hip.so"
+
+CLOSURES = {"MOST": 0, "CONSTANT": 1, "MOSTM": 2, "OAAHOC": 3}
+
+
+class Met(ctypes.Structure):
+    """Mirror of ``bldfm_met``."""
+
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("closure", ctypes.c_int32),
+        ("has_ustar", ctypes.c_int32),
+        ("has_z0", ctypes.c_int32),
+        ("has_domain_height", ctypes.c_int32),
+        ("has_stretch", ctypes.c_int32),
+        ("meas_height", ctypes.c_double),
+        ("um", ctypes.c_double),
+        ("vm", ctypes.c_double),
+        ("ustar", ctypes.c_double),
+        ("z0", ctypes.c_double),
+        ("mol", ctypes.c_double),
+        ("prsc", ctypes.c_double),
+        ("domain_height", ctypes.c_double),
+        ("stretch", ctypes.c_double),
+        ("tke", ctypes.c_double),
+    ]
+
+
+class Instance(ctypes.Structure):
+    """Mirror of ``bldfm_instance``."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nlx", ctypes.c_int32),
+        ("nly", ctypes.c_int32),
+        ("px", ctypes.c_int32),
+        ("py", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("nslot", ctypes.c_int32),
+        ("footprint", ctypes.c_int32),
+        ("analytic", ctypes.c_int32),
+        ("single", ctypes.c_int32),
+        ("out_f32", ctypes.c_int32),
+        ("has_shift", ctypes.c_int32),
+        ("prof_from_met", ctypes.c_int32),
+        ("fast", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("dx", ctypes.c_double),
+        ("dy", ctypes.c_double),
+        ("shift_x", ctypes.c_double),
+        ("shift_y", ctypes.c_double),
+        ("srf_bg_conc", ctypes.c_double),
+        ("met", Met),
+    ]
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+
+# name -> (restype, argtypes); every symbol the header declares
+SIGNATURES = {
+    "bldfm_abi_version": (ctypes.c_int, []),
+    "bldfm_last_error": (ctypes.c_char_p, []),
+    "bldfm_profile_query": (ctypes.c_int, [ctypes.POINTER(Met), ctypes.POINTER(_I32), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "bldfm_profiles": (ctypes.c_int, [ctypes.POINTER(Met), _P, _P]),
+    "bldfm_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Instance), _I32, ctypes.POINTER(_I32), ctypes.POINTER(_P)]),
+    "bldfm_plan_destroy": (ctypes.c_int, [_P]),
+    "bldfm_plan_sizes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "bldfm_plan_offsets": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
+    "bldfm_plan_execute": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "bldfm_stability": (ctypes.c_int, [_P, _I64, _P, _P, _P]),
+    "bldfm_wind_fields": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    "bldfm_math_probe": (ctypes.c_int, [_I32, _P, _P, _I64, _P, _P]),
+    "bldfm_fft2_workspace_bytes": (ctypes.c_size_t, [_I32, _I32, _I32]),
+    "bldfm_fft2": (ctypes.c_int, [ctypes.c_int, _I32, _I32, _I32, _P, _P, _I32, _D, _P, _P]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class BLDFMError(RuntimeError):
+    """Error reported by the HIP library."""
+
+
+def load():
+    """Load the HIP library (raises if it is missing: there is no fallback)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"BLDFM HIP library {LIB_PATH} is not built; run `python -m bldfm_amd.build`"
+            )
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc, what="BLDFM call"):
+    """Raise ValueError for argument errors, BLDFMError for the rest."""
+    if rc == 0:
+        return
+    msg = load().bldfm_last_error().decode(errors="replace")
+    if rc == -1:
+        raise ValueError(msg)
+    raise BLDFMError(f"{what} failed ({rc}): {msg}")
+
+
+def device():
+    """The torch device the solver runs on; raises without a HIP GPU."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("BLDFM needs a HIP GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle(dev=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height, stretch, tke):
+    """Fill a ``Met`` from vertical_profiles arguments (closure validated here)."""
+    if closure not in CLOSURES:
+        raise ValueError(
+            f"Invalid closure type: {closure}. "
+            "Supported closures are 'MOST', 'CONSTANT', and 'OAAHOC'."
+        )
+    um, vm = wind
+    m = Met()
+    m.n = int(n)
+    m.closure = CLOSURES[closure]
+    m.has_ustar = ustar is not None
+    m.has_z0 = z0 is not None
+    m.has_domain_height = domain_height is not None
+    m.has_stretch = stretch is not None
+    m.meas_height = float(meas_height)
+    m.um = float(um)
+    m.vm = float(vm)
+    m.ustar = float(ustar) if ustar is not None else 0.0
+    m.z0 = float(z0) if z0 is not None else 0.0
+    m.mol = float(mol)
+    m.prsc = float(prsc)
+    m.domain_height = float(domain_height) if domain_height is not None else 0.0
+    m.stretch = float(stretch) if stretch is not None else 0.0
+    m.tke = float(tke) if tke is not None else 1.0
+    return m
+
+
+class Plan:
+    """Owns a ``bldfm_plan``; sizes and per-instance offsets are cached."""
+
+    def __init__(self, instances, levels_i32, dev=None):
+        lib = load()
+        self.device = dev if dev is not None else device()
+        n = len(instances)
+        arr = (Instance * n)(*instances)
+        lv = (ctypes.c_int32 * len(levels_i32))(*[int(v) for v in levels_i32])
+        handle = ctypes.c_void_p()
+        check(lib.bldfm_plan_create(self.device.index, arr, n, lv, ctypes.byref(handle)), "plan_create")
+        self.handle = handle
+        ws = ctypes.c_size_t()
+        prof, src, out = _I64(), _I64(), _I64()
+        check(lib.bldfm_plan_sizes(handle, ctypes.byref(ws), ctypes.byref(prof), ctypes.byref(src), ctypes.byref(out)))
+        self.workspace_bytes = ws.value
+        self.prof_doubles = prof.value
+        self.src_doubles = src.value
+        self.out_elems = out.value
+        self.offsets = []
+        for i in range(n):
+            op, os_, oo, nz = _I64(), _I64(), _I64(), _I32()
+            check(lib.bldfm_plan_offsets(handle, i, ctypes.byref(op), ctypes.byref(os_), ctypes.byref(oo), ctypes.byref(nz)))
+            self.offsets.append((op.value, os_.value, oo.value, nz.value))
+
+    def execute(self, prof, src, conc, flx, workspace, stream=None):
+        lib = load()
+        s = stream if stream is not None else stream_handle(self.device)
+        check(
+            lib.bldfm_plan_execute(self.handle, ptr(prof), ptr(src), ptr(conc), ptr(flx), ptr(workspace), s),
+            "plan_execute",
+        )
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.bldfm_plan_destroy(h)
+            self.handle = None

@@ -1,0 +1,519 @@
+// Host side of the BLDFM C ABI: plan construction (validation, buffer
+// layout, tile tables, FFT factorisation and twiddles), execution, profiles
+// and the plain 2-D FFT.
+#include <math.h>
+#include <string.h>
+
+#include <cmath>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/bldfm_amd.h"
+#include "bldfm_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(BLDFM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = false;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+constexpr int kMaxE = BLDFM_FFT_MAXE;
+constexpr int kGridCap = 1 << 20;
+
+int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// ------------------------------------------------------------- FFT planning
+// Radix order: 8s, then one 4 or 2, then 3s, 5s, then remaining primes.
+bool factorize(int n, FftPlan* P) {
+  P->n = n;
+  P->nstage = 0;
+  int m = n;
+  auto push = [&](int r) {
+    if (P->nstage >= BLDFM_MAX_RADIX_STAGES) return false;
+    P->radix[P->nstage++] = r;
+    return true;
+  };
+  while (m % 8 == 0) { if (!push(8)) return false; m /= 8; }
+  if (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
+  if (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
+  for (int p = 3; m > 1; p += 2) {
+    while (m % p == 0) { if (!push(p)) return false; m /= p; }
+    if (p * p > m && m > 1) { if (!push(m)) return false; m = 1; }
+  }
+  return true;
+}
+
+// W_n^t = e^{-2 pi i t/n} in extended precision, with exact values on the
+// axes and exact conjugate symmetry W^{n-t} = conj W^t.
+void twiddles(int n, double2* w) {
+  const long double two_pi = 2.0L * acosl(-1.0L);
+  for (int t = 0; t <= n / 2; ++t) {
+    double c, s;
+    if ((4LL * t) % n == 0) {
+      const int q = (int)((4LL * t) / n);  // quarter turns
+      const double cs[4] = {1.0, 0.0, -1.0, 0.0};
+      const double sn[4] = {0.0, -1.0, 0.0, 1.0};
+      c = cs[q & 3];
+      s = sn[q & 3];
+    } else {
+      const long double a = two_pi * (long double)t / (long double)n;
+      c = (double)cosl(a);
+      s = (double)(-sinl(a));
+    }
+    w[t].x = c;
+    w[t].y = s;
+  }
+  for (int t = n / 2 + 1; t < n; ++t) {
+    w[t].x = w[n - t].x;
+    w[t].y = -w[n - t].y;
+  }
+}
+
+int fft_threads_for(int n) {
+  int t = (n + kMaxE - 1) / kMaxE;
+  t = (int)align_up(std::max(t, 64), 64);
+  return t;
+}
+
+int max_lds_bytes(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0)
+    v = 64 * 1024;
+  return v;
+}
+
+int check_fft_len(int n, int device, const char* what) {
+  const int64_t need = (int64_t)n * 16;
+  const int lds = max_lds_bytes(device);
+  if (need > lds || fft_threads_for(n) > 1024)
+    return fail(BLDFM_ELIMIT, std::string(what) + " = " + std::to_string(n) +
+                                  " exceeds the single-workgroup FFT limit (" +
+                                  std::to_string(std::min(lds / 16, 1024 * kMaxE)) + " points)");
+  return 0;
+}
+
+// Twiddle tables shared by plain 2-D FFT calls, per (device, n).
+std::mutex g_tw_mu;
+std::map<std::pair<int, int>, double2*> g_tw_cache;
+
+int cached_twiddles(int device, int n, const double2** out) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(device, n);
+  auto it = g_tw_cache.find(key);
+  if (it != g_tw_cache.end()) { *out = it->second; return 0; }
+  std::vector<double2> h(n);
+  twiddles(n, h.data());
+  double2* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(double2) * n);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
+  e = hipMemcpy(d, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { (void)hipFree(d); return hip_fail(e, "hipMemcpy(twiddles)"); }
+  g_tw_cache[key] = d;
+  *out = d;
+  return 0;
+}
+
+void set_lds_attr(int bytes) { bldfm_set_fft_lds_attr(bytes); }
+
+}  // namespace
+
+// ------------------------------------------------------------------- plan
+struct bldfm_plan {
+  int device = 0;
+  int ninst = 0;
+  std::vector<InstDesc> desc;
+  void* dmeta = nullptr;  // device metadata blob
+  PlanDev dev{};
+  size_t ws_bytes = 0;
+  int64_t tab_doubles = 0, ws_cplx = 0, prof_doubles = 0, src_doubles = 0, out_elems = 0;
+};
+
+extern "C" {
+
+int bldfm_abi_version(void) { return BLDFM_ABI_VERSION; }
+
+const char* bldfm_last_error(void) { return g_err.c_str(); }
+
+static int met_setup(const bldfm_met* m, ProfSetup* s) {
+  if (m->n <= 0) return fail(BLDFM_EINVAL, "n must be positive");
+  const int rc = prof_setup(m->n, m->meas_height, m->um, m->vm, m->has_ustar, m->ustar, m->has_z0,
+                            m->z0, m->mol, m->prsc, m->closure, m->has_domain_height,
+                            m->domain_height, m->has_stretch, m->stretch, m->tke, s);
+  if (rc == -1) return fail(BLDFM_EINVAL, "Either z0 or ustar must be provided.");
+  if (rc == -2) return fail(BLDFM_EINVAL, "Invalid closure type");
+  if (rc == -3) return fail(BLDFM_EINVAL, "vertical grid is empty");
+  if (!std::isfinite(s->z0) || !std::isfinite(s->ustar) || !std::isfinite(s->aa) ||
+      !std::isfinite(s->bb) || s->nlev < 2)
+    return fail(BLDFM_EINVAL, "profile parameters give a non-finite or degenerate grid");
+  return 0;
+}
+
+int bldfm_profile_query(const bldfm_met* met, int32_t* nlev, double* z0, double* ustar) {
+  if (!met) return fail(BLDFM_EINVAL, "null met");
+  ProfSetup s;
+  const int rc = met_setup(met, &s);
+  if (rc) return rc;
+  if (nlev) *nlev = s.nlev;
+  if (z0) *z0 = s.z0;
+  if (ustar) *ustar = s.ustar;
+  return 0;
+}
+
+int bldfm_profiles(const bldfm_met* met, double* prof6_dev, hipStream_t stream) {
+  if (!met || !prof6_dev) return fail(BLDFM_EINVAL, "null argument");
+  ProfSetup s;
+  const int rc = met_setup(met, &s);
+  if (rc) return rc;
+  const int lr = bldfm_launch_profiles(s, prof6_dev, stream);
+  if (lr) return fail(BLDFM_EHIP, std::string("profile kernel launch failed: ") +
+                                      hipGetErrorString((hipError_t)(-(lr + 1000))));
+  return 0;
+}
+
+int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, const int32_t* levels,
+                      bldfm_plan** out) {
+  if (!inst || ninst <= 0 || !levels || !out) return fail(BLDFM_EINVAL, "bad plan arguments");
+  *out = nullptr;
+  DeviceGuard guard(device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+
+  auto* P = new bldfm_plan();
+  P->device = device;
+  P->ninst = ninst;
+  P->desc.resize(ninst);
+  std::vector<ProfSetup> met(ninst);
+  std::vector<int32_t> met_flag(ninst, 0);
+  std::vector<int32_t> ints;
+  std::vector<int> plan_n;  // distinct transform lengths
+  auto plan_id = [&](int n) {
+    for (size_t i = 0; i < plan_n.size(); ++i)
+      if (plan_n[i] == n) return (int)i;
+    plan_n.push_back(n);
+    return (int)plan_n.size() - 1;
+  };
+  std::vector<int64_t> tiles[7];
+  for (auto& v : tiles) v.assign(ninst + 1, 0);
+  int64_t off_prof = 0, off_tab = 0, off_src = 0, off_ws = 0, off_out = 0;
+  const int32_t* lv = levels;
+  int out_f32 = -1, any_disp = 0, variant_used[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  int max_n = 1;
+
+  auto bad = [&](int code, const std::string& m) {
+    delete P;
+    return fail(code, m);
+  };
+
+  for (int i = 0; i < ninst; ++i) {
+    const bldfm_instance& I = inst[i];
+    InstDesc& D = P->desc[i];
+    memset(&D, 0, sizeof(D));
+    if (I.nx <= 0 || I.ny <= 0) return bad(BLDFM_EINVAL, "grid must be non-empty");
+    if (I.nlx <= 0 || I.nly <= 0) return bad(BLDFM_EINVAL, "modes must be positive");
+    if (I.px < 0 || I.py < 0) return bad(BLDFM_EINVAL, "halo pad widths must be non-negative");
+    if (I.nslot <= 0) return bad(BLDFM_EINVAL, "at least one output level is required");
+    D.nx = I.nx; D.ny = I.ny; D.nlx = I.nlx; D.nly = I.nly; D.px = I.px; D.py = I.py;
+    D.nxe = I.nx + 2 * I.px;
+    D.nye = I.ny + 2 * I.py;
+    if (D.nlx > D.nxe || D.nly > D.nye) return bad(BLDFM_EINVAL, "modes exceed the padded grid");
+    if ((D.nxe - D.nlx) % 2 || (D.nye - D.nly) % 2)
+      return bad(BLDFM_EINVAL, "padded grid and mode counts must have equal parity.");
+    int rc = check_fft_len(D.nxe, device, "padded x length");
+    if (!rc) rc = check_fft_len(D.nye, device, "padded y length");
+    if (rc) { delete P; return rc; }
+    max_n = std::max(max_n, std::max(D.nxe, D.nye));
+    D.nxb = band_len(D.nlx, D.nxe);
+    D.nyb = band_len(D.nly, D.nye);
+    if (I.prof_from_met) {
+      rc = met_setup(&I.met, &met[i]);
+      if (rc) { delete P; return rc; }
+      met_flag[i] = 1;
+      D.nz = met[i].nlev;
+    } else {
+      D.nz = I.nz;
+    }
+    if (D.nz < 2) return bad(BLDFM_EINVAL, "at least two vertical points are required");
+    D.nslot = I.nslot;
+    D.footprint = I.footprint ? 1 : 0;
+    D.analytic = I.analytic ? 1 : 0;
+    D.single = I.single ? 1 : 0;
+    D.has_shift = I.has_shift ? 1 : 0;
+    D.out_f32 = I.out_f32 ? 1 : 0;
+    if (out_f32 < 0) out_f32 = D.out_f32;
+    if (out_f32 != D.out_f32) return bad(BLDFM_EINVAL, "all instances of a plan must share out_f32");
+    if (!D.footprint) any_disp = 1;
+    D.plan_x = plan_id(D.nxe);
+    D.plan_y = plan_id(D.nye);
+    D.lanes = shoot_lanes(D.nlx, D.nly);
+    D.dx = I.dx; D.dy = I.dy;
+    if (!(I.dx > 0.0) || !(I.dy > 0.0)) return bad(BLDFM_EINVAL, "grid increments must be positive");
+    D.sx = 2.0 * BLDFM_PI / I.dx / D.nxe;
+    D.sy = 2.0 * BLDFM_PI / I.dy / D.nye;
+    D.vx = 1.0 / (D.nlx * (1.0 / D.nlx));
+    D.vy = 1.0 / (D.nly * (1.0 / D.nly));
+    D.shx = I.shift_x; D.shy = I.shift_y;
+    D.p000 = I.srf_bg_conc;
+    D.q00 = 1.0 / D.nxe / D.nye;
+
+    // levels: slot map then level index per slot
+    D.off_slot = (int64_t)ints.size();
+    int nfilled = 0, max_level = -1;
+    for (int z = 0; z < D.nz; ++z) {
+      const int s = lv[z];
+      if (s >= 0) {
+        if (s != nfilled || s >= D.nslot) return bad(BLDFM_EINVAL, "level->slot map must number slots 0,1,.. in ascending level order");
+        ++nfilled;
+        max_level = z;
+      }
+      ints.push_back(s);
+    }
+    for (int s = 0; s < D.nslot; ++s) {
+      const int li = lv[D.nz + s];
+      if (D.analytic && (li < 0 || li >= D.nz)) return bad(BLDFM_EINVAL, "analytic level index out of range");
+      ints.push_back(li);
+    }
+    lv += D.nz + D.nslot;
+    D.nfilled = nfilled;
+    D.max_level = max_level;
+    D.variant = D.analytic ? 3 : (nfilled <= 1 ? 0 : (nfilled <= 4 ? 1 : 2));
+    D.exact = I.fast ? 0 : 1;
+    variant_used[D.variant][D.exact] = 1;
+
+    // buffer offsets
+    D.off_prof = off_prof; off_prof += 6LL * D.nz;
+    D.off_tab = off_tab; off_tab += (int64_t)D.nz * BLDFM_LEVEL_STRIDE;
+    D.off_src = off_src;
+    if (!D.footprint) off_src += (int64_t)D.ny * D.nx;
+    if (!D.footprint) {
+      D.off_aw = off_ws; off_ws += (int64_t)D.ny * D.nlx;
+      D.off_q0 = off_ws; off_ws += (int64_t)D.nlx * D.nly;
+    } else {
+      D.off_aw = D.off_q0 = -1;
+    }
+    D.off_z = off_ws; off_ws += (int64_t)D.nslot * D.nxb * D.nyb;
+    D.off_nyq = off_ws; off_ws += (int64_t)D.nslot * (D.nlx + D.nly) * 2;
+    D.off_v = off_ws; off_ws += (int64_t)D.nslot * D.nxb * D.ny;
+    D.off_out = off_out; off_out += (int64_t)D.nslot * D.ny * D.nx;
+
+    // tiles: tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows
+    const int fixc = nyq_fix_count(D.nlx, D.nly, D.nxb, D.nyb);
+    const int64_t nt[7] = {
+        (D.nz + 255) / 256,
+        D.footprint ? 0 : D.ny,
+        D.footprint ? 0 : D.nlx,
+        (D.lanes + 255) / 256,
+        fixc > 0 ? (int64_t)D.nslot * ((fixc + 255) / 256) : 0,
+        (int64_t)D.nslot * D.nxb,
+        (int64_t)D.nslot * D.ny,
+    };
+    for (int k = 0; k < 7; ++k) tiles[k][i + 1] = tiles[k][i] + nt[k];
+  }
+  if (lv - levels > (int64_t)1 << 31) return bad(BLDFM_EINVAL, "level table too large");
+
+  // FFT plans and twiddles
+  std::vector<FftPlan> plans(plan_n.size());
+  int64_t tw_total = 0;
+  for (size_t k = 0; k < plan_n.size(); ++k) {
+    if (!factorize(plan_n[k], &plans[k])) return bad(BLDFM_ELIMIT, "too many FFT stages");
+    plans[k].tw_off = tw_total;
+    tw_total += plan_n[k];
+  }
+  std::vector<double2> tw(tw_total);
+  for (size_t k = 0; k < plan_n.size(); ++k) twiddles(plan_n[k], tw.data() + plans[k].tw_off);
+
+  // device metadata blob
+  size_t off = 0;
+  auto take = [&](size_t bytes) { const size_t o = off; off = (size_t)align_up((int64_t)(off + bytes), 256); return o; };
+  const size_t o_desc = take(sizeof(InstDesc) * ninst);
+  const size_t o_met = take(sizeof(ProfSetup) * ninst);
+  const size_t o_flag = take(sizeof(int32_t) * ninst);
+  const size_t o_ints = take(sizeof(int32_t) * std::max<size_t>(ints.size(), 1));
+  const size_t o_plans = take(sizeof(FftPlan) * plans.size());
+  const size_t o_tw = take(sizeof(double2) * tw_total);
+  size_t o_tiles[7];
+  for (int k = 0; k < 7; ++k) o_tiles[k] = take(sizeof(int64_t) * (ninst + 1));
+  std::vector<char> blob(off, 0);
+  memcpy(blob.data() + o_desc, P->desc.data(), sizeof(InstDesc) * ninst);
+  memcpy(blob.data() + o_met, met.data(), sizeof(ProfSetup) * ninst);
+  memcpy(blob.data() + o_flag, met_flag.data(), sizeof(int32_t) * ninst);
+  if (!ints.empty()) memcpy(blob.data() + o_ints, ints.data(), sizeof(int32_t) * ints.size());
+  memcpy(blob.data() + o_plans, plans.data(), sizeof(FftPlan) * plans.size());
+  memcpy(blob.data() + o_tw, tw.data(), sizeof(double2) * tw_total);
+  for (int k = 0; k < 7; ++k) memcpy(blob.data() + o_tiles[k], tiles[k].data(), sizeof(int64_t) * (ninst + 1));
+  hipError_t e = hipMalloc(&P->dmeta, off);
+  if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(plan metadata)"); }
+  e = hipMemcpy(P->dmeta, blob.data(), off, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { (void)hipFree(P->dmeta); delete P; return hip_fail(e, "hipMemcpy(plan metadata)"); }
+
+  char* base = (char*)P->dmeta;
+  PlanDev& V = P->dev;
+  V.desc = (const InstDesc*)(base + o_desc);
+  V.met = (const ProfSetup*)(base + o_met);
+  V.met_flag = (const int32_t*)(base + o_flag);
+  V.ints = (const int32_t*)(base + o_ints);
+  V.plans = (const FftPlan*)(base + o_plans);
+  V.tw = (const double2*)(base + o_tw);
+  V.t_tab = (const int64_t*)(base + o_tiles[0]);
+  V.t_fwd_rows = (const int64_t*)(base + o_tiles[1]);
+  V.t_fwd_cols = (const int64_t*)(base + o_tiles[2]);
+  V.t_shoot = (const int64_t*)(base + o_tiles[3]);
+  V.t_nyq = (const int64_t*)(base + o_tiles[4]);
+  V.t_inv_cols = (const int64_t*)(base + o_tiles[5]);
+  V.t_inv_rows = (const int64_t*)(base + o_tiles[6]);
+  V.ninst = ninst;
+  V.any_dispersion = any_disp;
+  for (int k = 0; k < 4; ++k)
+    for (int e = 0; e < 2; ++e) V.variant_used[k][e] = variant_used[k][e];
+  V.fft_threads = fft_threads_for(max_n);
+  V.fft_lds_bytes = max_n * (int)sizeof(double2);
+  V.tot_tab = tiles[0][ninst];
+  V.tot_fwd_rows = tiles[1][ninst];
+  V.tot_fwd_cols = tiles[2][ninst];
+  V.tot_shoot = tiles[3][ninst];
+  V.tot_nyq = tiles[4][ninst];
+  V.tot_inv_cols = tiles[5][ninst];
+  V.tot_inv_rows = tiles[6][ninst];
+  set_lds_attr(V.fft_lds_bytes);
+
+  P->tab_doubles = off_tab;
+  P->ws_cplx = off_ws;
+  P->prof_doubles = off_prof;
+  P->src_doubles = off_src;
+  P->out_elems = off_out;
+  P->ws_bytes = (size_t)align_up(off_tab * 8, 256) + (size_t)off_ws * 16;
+  *out = P;
+  return 0;
+}
+
+int bldfm_plan_destroy(bldfm_plan* plan) {
+  if (!plan) return 0;
+  if (plan->dmeta) {
+    DeviceGuard g(plan->device);
+    (void)hipFree(plan->dmeta);
+  }
+  delete plan;
+  return 0;
+}
+
+int bldfm_plan_sizes(const bldfm_plan* P, size_t* ws, int64_t* prof, int64_t* src, int64_t* out) {
+  if (!P) return fail(BLDFM_EINVAL, "null plan");
+  if (ws) *ws = P->ws_bytes;
+  if (prof) *prof = P->prof_doubles;
+  if (src) *src = P->src_doubles;
+  if (out) *out = P->out_elems;
+  return 0;
+}
+
+int bldfm_plan_offsets(const bldfm_plan* P, int32_t i, int64_t* off_prof, int64_t* off_src,
+                       int64_t* off_out, int32_t* nz) {
+  if (!P || i < 0 || i >= P->ninst) return fail(BLDFM_EINVAL, "bad plan or instance index");
+  const InstDesc& D = P->desc[i];
+  if (off_prof) *off_prof = D.off_prof;
+  if (off_src) *off_src = D.off_src;
+  if (off_out) *off_out = D.off_out;
+  if (nz) *nz = D.nz;
+  return 0;
+}
+
+int bldfm_plan_execute(bldfm_plan* P, double* prof, const double* src, void* conc, void* flx,
+                       void* workspace, hipStream_t stream) {
+  if (!P) return fail(BLDFM_EINVAL, "null plan");
+  if (!prof || !conc || !flx || !workspace) return fail(BLDFM_EINVAL, "null buffer");
+  if (P->dev.any_dispersion && !src) return fail(BLDFM_EINVAL, "dispersion instances need srf_flx");
+  DeviceGuard guard(P->device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  ExecBufs B;
+  B.prof = prof;
+  B.src = src;
+  B.conc = conc;
+  B.flx = flx;
+  B.tab = (double*)workspace;
+  B.ws = (double2*)((char*)workspace + align_up(P->tab_doubles * 8, 256));
+  const int rc = bldfm_launch_all(P->dev, B, kGridCap, stream);
+  if (rc) return fail(BLDFM_EHIP, std::string("kernel launch failed: ") +
+                                      hipGetErrorString((hipError_t)(-(rc + 1000))));
+  return 0;
+}
+
+int bldfm_stability(const double* x, int64_t n, double* psi, double* phi, hipStream_t stream) {
+  if (n < 0 || (n > 0 && !x)) return fail(BLDFM_EINVAL, "bad stability arguments");
+  const int rc = bldfm_launch_stability(x, n, psi, phi, stream);
+  if (rc) return fail(BLDFM_EHIP, std::string("stability kernel launch failed: ") +
+                                      hipGetErrorString((hipError_t)(-(rc + 1000))));
+  return 0;
+}
+
+int bldfm_wind_fields(const double* speed, const double* dir, int64_t n, double* u, double* v,
+                      hipStream_t stream) {
+  if (n < 0 || (n > 0 && (!speed || !dir || !u || !v))) return fail(BLDFM_EINVAL, "bad wind arguments");
+  const int rc = bldfm_launch_wind(speed, dir, n, u, v, stream);
+  if (rc) return fail(BLDFM_EHIP, std::string("wind kernel launch failed: ") +
+                                      hipGetErrorString((hipError_t)(-(rc + 1000))));
+  return 0;
+}
+
+int bldfm_math_probe(int32_t op, const double* a, const double* b, int64_t n, double* out,
+                     hipStream_t stream) {
+  if (op < 0 || op > 5 || n < 0 || (n > 0 && (!a || !out || (op != 5 && !b))))
+    return fail(BLDFM_EINVAL, "bad math probe arguments");
+  const int rc = bldfm_launch_math_probe(op, a, b, n, out, stream);
+  if (rc) return fail(BLDFM_EHIP, std::string("math probe launch failed: ") +
+                                      hipGetErrorString((hipError_t)(-(rc + 1000))));
+  return 0;
+}
+
+size_t bldfm_fft2_workspace_bytes(int32_t n0, int32_t n1, int32_t batch) {
+  return (size_t)n0 * n1 * batch * sizeof(double2);
+}
+
+int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* in, double* out,
+               int32_t sign, double scale, void* workspace, hipStream_t stream) {
+  if (n0 <= 0 || n1 <= 0 || batch <= 0 || !in || !out || !workspace)
+    return fail(BLDFM_EINVAL, "bad fft2 arguments");
+  if (sign != 1 && sign != -1) return fail(BLDFM_EINVAL, "sign must be +1 or -1");
+  DeviceGuard guard(device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  int rc = check_fft_len(n0, device, "fft2 axis 0 length");
+  if (!rc) rc = check_fft_len(n1, device, "fft2 axis 1 length");
+  if (rc) return rc;
+  FftPlan F0, F1;
+  if (!factorize(n0, &F0) || !factorize(n1, &F1)) return fail(BLDFM_ELIMIT, "too many FFT stages");
+  F0.tw_off = F1.tw_off = 0;
+  const double2 *tw0 = nullptr, *tw1 = nullptr;
+  rc = cached_twiddles(device, n0, &tw0);
+  if (!rc) rc = cached_twiddles(device, n1, &tw1);
+  if (rc) return rc;
+  const int nmax = std::max(n0, n1);
+  set_lds_attr(nmax * 16);
+  rc = bldfm_launch_fft2(F0, tw0, F1, tw1, batch, (const double2*)in, (double2*)workspace,
+                         (double2*)out, sign, scale, fft_threads_for(nmax), nmax * 16, kGridCap, stream);
+  if (rc) return fail(BLDFM_EHIP, std::string("fft2 launch failed: ") +
+                                      hipGetErrorString((hipError_t)(-(rc + 1000))));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,681 @@
+// HIP kernels of the BLDFM steady-state solver for gfx950 (MI355X).
+//
+// Pipeline per solve instance (all FP64):
+//   k_level_table  per-step constants of the vertical march (profiles from
+//                  caller arrays, or evaluated here from met scalars)
+//   k_fwd_rows/cols  dispersion mode only: pruned forward 2-D DFT of the
+//                  halo-padded surface flux onto the retained modes
+//   k_shoot_*      one lane per Fourier mode: two initial-value problems,
+//                  radiation condition, shooting, (0,0) trapezoid, phase
+//                  shift, Hermitian packing of p^ and q^ into Z = Hp + i Hq
+//   k_nyq_fix      Hermitian part on the unpaired Nyquist row/column
+//   k_inv_cols/rows  pruned inverse 2-D DFT of Z: y-pass over the band
+//                  columns, x-pass over the cropped rows; Re -> conc, Im -> flx
+//
+// Work of a batch of instances is laid out as tiles with per-kernel prefix
+// sums; every kernel is a grid-stride loop over its tiles.
+#include "bldfm_internal.h"
+#include "bldfm_fft.h"
+
+#define SHOOT_THREADS 256
+
+__device__ __forceinline__ int find_inst(const int64_t* __restrict__ off, int ninst, int64_t t) {
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- level table
+// Entry i < nz-1 (step from z_i to z_{i+1}, coefficients of the LOWER level,
+// src/bldfm/solver.py:357-364), with kappa = 1/Kz_i, h = z_{i+1} - z_i:
+//   [0..3]  Kx, Ky, u, v
+//   [4..11] the reference's scalar sub-expressions, rounded as numba rounds
+//           them: sa = 0.5*kappa, h2 = h*h, b0 = (-kappa)*h,
+//           sb = (1/6)*(kappa*kappa), h3 = h*(h*h), h, sc = (1/6)*kappa,
+//           W = 0.5/Kz_i + 0.5/Kz_{i+1}  (trapezoid weight of the (0,0) mode)
+//   [12..14] fused-path coefficients A = sa*h2, B1 = sb*h3, C = sc*h3
+// so that T = -(Kx lx^2 + Ky ly^2) - i(u lx + v ly), a = d = 1 - sa T h2,
+// b = b0 - sb T h3, c = T h - sc T^2 h3.
+// Entry nz-1 (top, src/bldfm/solver.py:164-174):
+//   {Kx/Kz, Ky/Kz, u/Kz, v/Kz, Kz, 1/Kz} (products with 1/Kz as the reference forms them)
+__global__ __launch_bounds__(256) void k_level_table(PlanDev P, ExecBufs B) {
+#pragma clang fp contract(off)
+  for (int64_t t = blockIdx.x; t < P.tot_tab; t += gridDim.x) {
+    const int inst = find_inst(P.t_tab, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    const int nz = D.nz;
+    const int i = (int)(t - P.t_tab[inst]) * blockDim.x + threadIdx.x;
+    if (i >= nz) continue;
+    double* prof = B.prof + D.off_prof;
+    double cur[6], nxt[6];
+    if (P.met_flag[inst]) {
+      const ProfSetup& S = P.met[inst];
+      prof_point(S, i, cur);
+      for (int c = 0; c < 6; ++c) prof[(int64_t)c * nz + i] = cur[c];
+      if (i + 1 < nz) prof_point(S, i + 1, nxt);
+    } else {
+      for (int c = 0; c < 6; ++c) cur[c] = prof[(int64_t)c * nz + i];
+      if (i + 1 < nz)
+        for (int c = 0; c < 6; ++c) nxt[c] = prof[(int64_t)c * nz + i + 1];
+    }
+    double* T = B.tab + D.off_tab + (int64_t)i * BLDFM_LEVEL_STRIDE;
+    const double kz = cur[5];
+    const double kinv = 1.0 / kz;
+    if (i < nz - 1) {
+      const double h = nxt[0] - cur[0];
+      const double h2 = h * h;
+      const double h3 = h * h2;
+      const double sixth = 1.0 / 6.0;
+      const double sa = 0.5 * kinv, sb = sixth * (kinv * kinv), sc = sixth * kinv;
+      T[0] = cur[3];
+      T[1] = cur[4];
+      T[2] = cur[1];
+      T[3] = cur[2];
+      T[4] = sa;
+      T[5] = h2;
+      T[6] = -kinv * h;
+      T[7] = sb;
+      T[8] = h3;
+      T[9] = h;
+      T[10] = sc;
+      T[11] = 0.5 / kz + 0.5 / nxt[5];
+      T[12] = sa * h2;
+      T[13] = sb * h3;
+      T[14] = sc * h3;
+      T[15] = 0.0;
+    } else {
+      T[0] = cur[3] * kinv;
+      T[1] = cur[4] * kinv;
+      T[2] = cur[1] * kinv;
+      T[3] = cur[2] * kinv;
+      T[4] = kz;
+      T[5] = kinv;
+      for (int c = 6; c < BLDFM_LEVEL_STRIDE; ++c) T[c] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- shooting
+struct Lane {
+  int kind, fx, fy, nidx;
+  double lx, ly;
+};
+
+__device__ __forceinline__ bool lane_setup(const InstDesc& D, int lane, Lane& L) {
+  if (lane >= D.lanes) return false;
+  L.fx = 0; L.fy = 0; L.nidx = -1;
+  L.kind = decode_lane(lane, D.nlx, D.nly, &L.fx, &L.fy, &L.nidx);
+  if (L.kind == LANE_IDLE) return false;
+  // lx = ((2 pi)/dx)/nxe * (f * 1/(nl*(1/nl)))   (numpy fftfreq, solver.py:148-153)
+  L.lx = xmul(D.sx, xmul((double)L.fx, D.vx));
+  L.ly = xmul(D.sy, xmul((double)L.fy, D.vy));
+  return true;
+}
+
+// Packed output of one mode (after complex64 rounding and phase shift).
+__device__ __forceinline__ void emit_mode(const InstDesc& D, const Lane& L, double2* __restrict__ ws,
+                                          int s, double2 Xp, double2 Xq, double2 sh) {
+  if (D.single) { Xp = round_c64(Xp); Xq = round_c64(Xq); }
+  if (D.has_shift) { Xp = ncmul(Xp, sh); Xq = ncmul(Xq, sh); }
+  if (L.kind == LANE_HALF) {
+    const int cx = compact_of_freq(L.fx, D.nxb), cy = compact_of_freq(L.fy, D.nyb);
+    const int mx = compact_mirror(cx, D.nxb), my = compact_mirror(cy, D.nyb);
+    double2* Z = ws + D.off_z + (int64_t)s * D.nxb * D.nyb;
+    // Z(k) = Xp + i Xq ; Z(-k) = conj(Xp) + i conj(Xq)
+    Z[(int64_t)cx * D.nyb + cy] = cmk(Xp.x - Xq.y, Xp.y + Xq.x);
+    Z[(int64_t)mx * D.nyb + my] = cmk(Xp.x + Xq.y, Xq.x - Xp.y);
+  } else {  // LANE_NYQ: keep X for the Hermitian fix-up
+    double2* N = ws + D.off_nyq + ((int64_t)s * (D.nlx + D.nly) + L.nidx) * 2;
+    N[0] = Xp;
+    N[1] = Xq;
+  }
+}
+
+__device__ __forceinline__ void emit_origin(const InstDesc& D, double2* __restrict__ ws, int s,
+                                            double p00, double q00) {
+  if (D.single) { p00 = (double)(float)p00; q00 = (double)(float)q00; }
+  ws[D.off_z + (int64_t)s * D.nxb * D.nyb] = cmk(p00, q00);
+}
+
+// e^{i(lx*shx + ly*shy)} (src/bldfm/solver.py:254-262)
+__device__ __forceinline__ double2 phase_shift(const InstDesc& D, const Lane& L) {
+  double s = 0.0, c = 1.0;
+  if (D.has_shift) {
+    const double th = xmul(L.lx, D.shx) + xmul(L.ly, D.shy);
+    sincos(th, &s, &c);
+  }
+  return cmk(c, s);
+}
+
+__device__ __forceinline__ double2 ivp2_init(const InstDesc& D, const Lane& L,
+                                             const double2* __restrict__ ws) {
+  if (D.footprint) return cmk(D.q00, 0.0);
+  const int jx = trunc_of_freq(L.fx, D.nlx), jy = trunc_of_freq(L.fy, D.nly);
+  return ws[D.off_q0 + (int64_t)jx * D.nly + jy];
+}
+
+// One step of both initial-value problems,
+// [p; q] <- [[a, b], [c, a]] [p; q]  (src/bldfm/solver.py:357-368).
+// EXACT: every product and sum rounded separately, in the reference's order
+//   T = (-(Kx*lx2 + Ky*ly2), -(u*lx + v*ly))
+//   a = 1 - (sa*T)*h2,  b = b0 - (sb*T)*h3,  c = T*h - (sc*T^2)*h3
+// otherwise: the same algebra with fused multiply-adds.
+template <bool EXACT>
+__device__ __forceinline__ void march_step(const double* __restrict__ T, double lx, double ly,
+                                           double lx2, double ly2, double2& p1, double2& q1,
+                                           double2& p2, double2& q2);
+
+template <>
+__device__ __forceinline__ void march_step<true>(const double* __restrict__ T, double lx, double ly,
+                                                 double lx2, double ly2, double2& p1, double2& q1,
+                                                 double2& p2, double2& q2) {
+#pragma clang fp contract(off)
+  const double kx = T[0], ky = T[1], u = T[2], v = T[3];
+  const double sa = T[4], h2 = T[5], b0 = T[6], sb = T[7], h3 = T[8], h = T[9], sc = T[10];
+  const double tr = -(kx * lx2 + ky * ly2);
+  const double ti = -(u * lx + v * ly);
+  const double ar = 1.0 - (sa * tr) * h2, ai = -((sa * ti) * h2);
+  const double br = b0 - (sb * tr) * h3, bi = -((sb * ti) * h3);
+  const double t2r = tr * tr - ti * ti;
+  const double tri = tr * ti;
+  const double t2i = tri + tri;
+  const double cr = tr * h - (sc * t2r) * h3, ci = ti * h - (sc * t2i) * h3;
+  const double2 a = cmk(ar, ai), b = cmk(br, bi), c = cmk(cr, ci);
+  const double2 np1 = xcmuladd(a, p1, b, q1);
+  const double2 nq1 = xcmuladd(c, p1, a, q1);
+  const double2 np2 = xcmuladd(a, p2, b, q2);
+  const double2 nq2 = xcmuladd(c, p2, a, q2);
+  p1 = np1; q1 = nq1; p2 = np2; q2 = nq2;
+}
+
+template <>
+__device__ __forceinline__ void march_step<false>(const double* __restrict__ T, double lx, double ly,
+                                                  double lx2, double ly2, double2& p1, double2& q1,
+                                                  double2& p2, double2& q2) {
+  const double tr = -(T[0] * lx2 + T[1] * ly2);
+  const double ti = -(T[2] * lx + T[3] * ly);
+  const double A = T[12], B0 = T[6], B1 = T[13], H = T[9], C = T[14];
+  const double2 a = cmk(1.0 - A * tr, -(A * ti));
+  const double2 b = cmk(B0 - B1 * tr, -(B1 * ti));
+  const double t2r = tr * tr - ti * ti, t2i = 2.0 * tr * ti;
+  const double2 c = cmk(H * tr - C * t2r, H * ti - C * t2i);
+  const double2 np1 = cadd(cmul(a, p1), cmul(b, q1));
+  const double2 nq1 = cadd(cmul(c, p1), cmul(a, q1));
+  const double2 np2 = cadd(cmul(a, p2), cmul(b, q2));
+  const double2 nq2 = cadd(cmul(c, p2), cmul(a, q2));
+  p1 = np1; q1 = nq1; p2 = np2; q2 = nq2;
+}
+
+// Radiation condition q = Kz lambda p at the top and the shooting coefficient
+// alpha = -(q2 - Kz lambda p2)/(q1 - Kz lambda p1)
+// (src/bldfm/solver.py:164-174, 228-230), rounded as numpy rounds it.
+__device__ __forceinline__ double2 top_lambda(const double* __restrict__ tt, double lx, double ly,
+                                              double lx2, double ly2) {
+#pragma clang fp contract(off)
+  return csqrt_exact(cmk(tt[0] * lx2 + tt[1] * ly2, tt[2] * lx + tt[3] * ly));
+}
+
+__device__ __forceinline__ double2 shoot_alpha(const double* __restrict__ tt, double2 lam, double2 p1,
+                                               double2 q1, double2 p2, double2 q2) {
+  const double2 kl = xcscale(tt[4], lam);
+  const double2 num = xcsub(q2, ncmul(kl, p2));
+  const double2 den = xcsub(q1, ncmul(kl, p1));
+  return cdiv(cmk(-num.x, -num.y), den);
+}
+
+// Degenerate mode (0,0): flux conserved, p by the trapezoid rule in z
+// (src/bldfm/solver.py:190-191, 239-251): p00 <- p00 - (q00*dz)*W.
+__device__ void origin_numeric(const InstDesc& D, const double* __restrict__ tab,
+                               const int32_t* __restrict__ slot, double2* __restrict__ ws) {
+#pragma clang fp contract(off)
+  const double q00 = D.footprint ? D.q00 : ws[D.off_q0].x;
+  double p00 = D.p000;
+  for (int i = 0; i < D.nz; ++i) {
+    const int sl = slot[i];
+    if (sl >= 0) emit_origin(D, ws, sl, p00, q00);
+    if (i < D.nz - 1) {
+      const double* T = tab + (int64_t)i * BLDFM_LEVEL_STRIDE;
+      p00 = p00 - (q00 * T[9]) * T[11];
+    }
+  }
+  for (int s = D.nfilled; s < D.nslot; ++s) emit_origin(D, ws, s, s == 0 ? D.p000 : 0.0, q00);
+}
+
+// Numerical branch.  MAXS register snapshots (RECOMP == false), or a second
+// march from the bottom after the shooting coefficient is known
+// (RECOMP == true) for many output levels.  Both give the same values.
+template <int MAXS, bool RECOMP, bool EXACT>
+__global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, int variant) {
+  for (int64_t t = blockIdx.x; t < P.tot_shoot; t += gridDim.x) {
+    const int inst = find_inst(P.t_shoot, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    if (D.variant != variant || D.exact != (int)EXACT) continue;
+    Lane L;
+    const int lane = (int)(t - P.t_shoot[inst]) * SHOOT_THREADS + threadIdx.x;
+    if (!lane_setup(D, lane, L)) continue;
+    const double* __restrict__ tab = B.tab + D.off_tab;
+    const int32_t* __restrict__ slot = P.ints + D.off_slot;
+    const int nz = D.nz;
+    if (L.kind == LANE_ORIGIN) {
+      origin_numeric(D, tab, slot, B.ws);
+      continue;
+    }
+    const double lx = L.lx, ly = L.ly, lx2 = xmul(lx, lx), ly2 = xmul(ly, ly);
+    const double2 q0 = ivp2_init(D, L, B.ws);
+    double2 p1 = cmk(1.0, 0.0), q1 = cmk(0.0, 0.0), p2 = cmk(0.0, 0.0), q2 = q0;
+    double2 sp1[MAXS], sq1[MAXS], sp2[MAXS], sq2[MAXS];
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s) sp1[s] = sq1[s] = sp2[s] = sq2[s] = cmk(0.0, 0.0);
+    for (int i = 0; i < nz - 1; ++i) {
+      if (!RECOMP) {
+        const int sl = slot[i];
+        if (sl >= 0) {
+#pragma unroll
+          for (int s = 0; s < MAXS; ++s)
+            if (s == sl) { sp1[s] = p1; sq1[s] = q1; sp2[s] = p2; sq2[s] = q2; }
+        }
+      }
+      march_step<EXACT>(tab + (int64_t)i * BLDFM_LEVEL_STRIDE, lx, ly, lx2, ly2, p1, q1, p2, q2);
+    }
+    if (!RECOMP) {
+      const int sl = slot[nz - 1];
+      if (sl >= 0) {
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s)
+          if (s == sl) { sp1[s] = p1; sq1[s] = q1; sp2[s] = p2; sq2[s] = q2; }
+      }
+    }
+    const double* tt = tab + (int64_t)(nz - 1) * BLDFM_LEVEL_STRIDE;
+    const double2 alpha = shoot_alpha(tt, top_lambda(tt, lx, ly, lx2, ly2), p1, q1, p2, q2);
+    const double2 sh = phase_shift(D, L);
+    if (!RECOMP) {
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s)
+        if (s < D.nfilled)
+          emit_mode(D, L, B.ws, s, xcadd(ncmul(alpha, sp1[s]), sp2[s]),
+                    xcadd(ncmul(alpha, sq1[s]), sq2[s]), sh);
+    } else {
+      p1 = cmk(1.0, 0.0); q1 = cmk(0.0, 0.0); p2 = cmk(0.0, 0.0); q2 = q0;
+      for (int i = 0; i <= D.max_level; ++i) {
+        const int sl = slot[i];
+        if (sl >= 0)
+          emit_mode(D, L, B.ws, sl, xcadd(ncmul(alpha, p1), p2), xcadd(ncmul(alpha, q1), q2), sh);
+        if (i < D.max_level)
+          march_step<EXACT>(tab + (int64_t)i * BLDFM_LEVEL_STRIDE, lx, ly, lx2, ly2, p1, q1, p2, q2);
+      }
+    }
+    for (int s = D.nfilled; s < D.nslot; ++s) emit_mode(D, L, B.ws, s, cmk(0.0, 0.0), cmk(0.0, 0.0), sh);
+  }
+}
+
+// Analytic branch for constant profiles (src/bldfm/solver.py:193-202):
+// q^ = q^0 e^{-lambda h}, p^ = (q^ / Kz) / lambda, p^00 = p000 - (q^00 / Kz) h.
+__global__ __launch_bounds__(SHOOT_THREADS) void k_shoot_analytic(PlanDev P, ExecBufs B, int variant) {
+#pragma clang fp contract(off)
+  for (int64_t t = blockIdx.x; t < P.tot_shoot; t += gridDim.x) {
+    const int inst = find_inst(P.t_shoot, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    if (D.variant != variant) continue;
+    Lane L;
+    const int lane = (int)(t - P.t_shoot[inst]) * SHOOT_THREADS + threadIdx.x;
+    if (!lane_setup(D, lane, L)) continue;
+    const int nz = D.nz;
+    const double* tt = B.tab + D.off_tab + (int64_t)(nz - 1) * BLDFM_LEVEL_STRIDE;
+    const double kinv = tt[5];
+    const double* z = B.prof + D.off_prof;
+    const int32_t* lvidx = P.ints + D.off_slot + nz;
+    if (L.kind == LANE_ORIGIN) {
+      const double q00 = D.footprint ? D.q00 : B.ws[D.off_q0].x;
+      for (int s = 0; s < D.nslot; ++s) {
+        const double h = z[lvidx[s]] - z[0];
+        emit_origin(D, B.ws, s, D.p000 - (q00 * kinv) * h, q00);
+      }
+      continue;
+    }
+    const double lx = L.lx, ly = L.ly;
+    const double2 lam = top_lambda(tt, lx, ly, lx * lx, ly * ly);
+    const double2 q0 = ivp2_init(D, L, B.ws);
+    const double2 sh = phase_shift(D, L);
+    for (int s = 0; s < D.nslot; ++s) {
+      const double h = z[lvidx[s]] - z[0];
+      double2 Xq = ncmul(q0, cexp_(cmk(-lam.x * h, -lam.y * h)));
+      if (D.single) Xq = round_c64(Xq);
+      const double2 Xp = cdiv(xcscale(kinv, Xq), lam);
+      emit_mode(D, L, B.ws, s, Xp, Xq, sh);
+    }
+  }
+}
+
+// Hermitian part on the Nyquist row/column:
+// Z(c) = (X(c) + conj X(-c))/2 packed as Hp + i Hq.
+__global__ __launch_bounds__(256) void k_nyq_fix(PlanDev P, ExecBufs B) {
+  for (int64_t t = blockIdx.x; t < P.tot_nyq; t += gridDim.x) {
+    const int inst = find_inst(P.t_nyq, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    const int cnt = nyq_fix_count(D.nlx, D.nly, D.nxb, D.nyb);
+    const int bps = (cnt + 255) / 256;
+    const int local = (int)(t - P.t_nyq[inst]);
+    const int s = local / bps;
+    const int p = (local % bps) * 256 + threadIdx.x;
+    if (p >= cnt) continue;
+    int cx, cy;
+    nyq_fix_decode(p, D.nlx, D.nly, D.nxb, D.nyb, &cx, &cy);
+    const int mx = compact_mirror(cx, D.nxb), my = compact_mirror(cy, D.nyb);
+    const int n1 = nyq_index_at(cx, cy, D.nlx, D.nly, D.nxb, D.nyb);
+    const int n2 = nyq_index_at(mx, my, D.nlx, D.nly, D.nxb, D.nyb);
+    const double2* N = B.ws + D.off_nyq + (int64_t)s * (D.nlx + D.nly) * 2;
+    double2 A = cmk(0.0, 0.0), Q = cmk(0.0, 0.0);
+    if (n1 >= 0) { A = N[2 * n1]; Q = N[2 * n1 + 1]; }
+    if (n2 >= 0) { A = cadd(A, cconj(N[2 * n2])); Q = cadd(Q, cconj(N[2 * n2 + 1])); }
+    B.ws[D.off_z + ((int64_t)s * D.nxb + cx) * D.nyb + cy] = cmk(0.5 * (A.x - Q.y), 0.5 * (A.y + Q.x));
+  }
+}
+
+// ---------------------------------------------------------------- FFT passes
+extern __shared__ double2 smem[];
+
+__device__ __forceinline__ void lds_zero(int n, int tid, int T) {
+  for (int i = tid; i < n; i += T) smem[i] = cmk(0.0, 0.0);
+  __syncthreads();
+}
+
+// Forward, rows: one surface-flux row y placed at [px, px+nx) of nxe points,
+// e^{-i} DFT, scale 1/nxe (numpy norm="forward"), keep the nlx retained
+// columns in truncated order.
+__global__ void k_fwd_rows(PlanDev P, ExecBufs B) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < P.tot_fwd_rows; t += gridDim.x) {
+    const int inst = find_inst(P.t_fwd_rows, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    const int y = (int)(t - P.t_fwd_rows[inst]);
+    const FftPlan& F = P.plans[D.plan_x];
+    lds_zero(F.n, tid, T);
+    const double* row = B.src + D.off_src + (int64_t)y * D.nx;
+    for (int x = tid; x < D.nx; x += T) smem[D.px + x] = cmk(row[x], 0.0);
+    __syncthreads();
+    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, false, tid, T);
+    const double fct = 1.0 / (double)F.n;
+    double2* aw = B.ws + D.off_aw + (int64_t)y * D.nlx;
+    for (int j = tid; j < D.nlx; j += T)
+      aw[j] = cscale(fct, smem[padded_of_freq(freq_of_trunc(j, D.nlx), F.n)]);
+    __syncthreads();
+  }
+}
+
+// Forward, columns: retained column jx over y placed at [py, py+ny) of nye
+// points, e^{-i} DFT, scale 1/nye, keep the nly retained rows -> q^0 (x-major).
+__global__ void k_fwd_cols(PlanDev P, ExecBufs B) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < P.tot_fwd_cols; t += gridDim.x) {
+    const int inst = find_inst(P.t_fwd_cols, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    const int jx = (int)(t - P.t_fwd_cols[inst]);
+    const FftPlan& F = P.plans[D.plan_y];
+    lds_zero(F.n, tid, T);
+    const double2* aw = B.ws + D.off_aw;
+    for (int y = tid; y < D.ny; y += T) smem[D.py + y] = aw[(int64_t)y * D.nlx + jx];
+    __syncthreads();
+    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, false, tid, T);
+    const double fct = 1.0 / (double)F.n;
+    double2* q0 = B.ws + D.off_q0 + (int64_t)jx * D.nly;
+    for (int j = tid; j < D.nly; j += T)
+      q0[j] = cscale(fct, smem[padded_of_freq(freq_of_trunc(j, D.nly), F.n)]);
+    __syncthreads();
+  }
+}
+
+// Inverse, y-pass: band column cx of slot s (compact rows scattered to their
+// padded indices), unnormalised DFT (e^{-i} footprint, e^{+i} dispersion),
+// keep the cropped rows [py, py+ny).
+__global__ void k_inv_cols(PlanDev P, ExecBufs B) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < P.tot_inv_cols; t += gridDim.x) {
+    const int inst = find_inst(P.t_inv_cols, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    const int64_t local = t - P.t_inv_cols[inst];  // s*nxb + cx
+    const FftPlan& F = P.plans[D.plan_y];
+    lds_zero(F.n, tid, T);
+    const double2* zc = B.ws + D.off_z + local * D.nyb;
+    for (int c = tid; c < D.nyb; c += T) smem[padded_of_compact(c, D.nyb, F.n)] = zc[c];
+    __syncthreads();
+    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
+    double2* v = B.ws + D.off_v + local * D.ny;
+    for (int y = tid; y < D.ny; y += T) v[y] = smem[D.py + y];
+    __syncthreads();
+  }
+}
+
+// Inverse, x-pass: cropped row y of slot s over the band columns, DFT,
+// keep [px, px+nx); conc = Re, flx = Im.
+__global__ void k_inv_rows(PlanDev P, ExecBufs B) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < P.tot_inv_rows; t += gridDim.x) {
+    const int inst = find_inst(P.t_inv_rows, P.ninst, t);
+    const InstDesc& D = P.desc[inst];
+    const int64_t local = t - P.t_inv_rows[inst];  // s*ny + y
+    const int s = (int)(local / D.ny), y = (int)(local % D.ny);
+    const FftPlan& F = P.plans[D.plan_x];
+    lds_zero(F.n, tid, T);
+    const double2* v = B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y;
+    for (int c = tid; c < D.nxb; c += T) smem[padded_of_compact(c, D.nxb, F.n)] = v[(int64_t)c * D.ny];
+    __syncthreads();
+    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
+    const int64_t o = D.off_out + local * D.nx;
+    if (D.out_f32) {
+      float* conc = (float*)B.conc + o;
+      float* flx = (float*)B.flx + o;
+      for (int x = tid; x < D.nx; x += T) {
+        const double2 w = smem[D.px + x];
+        conc[x] = (float)w.x;
+        flx[x] = (float)w.y;
+      }
+    } else {
+      double* conc = (double*)B.conc + o;
+      double* flx = (double*)B.flx + o;
+      for (int x = tid; x < D.nx; x += T) {
+        const double2 w = smem[D.px + x];
+        conc[x] = w.x;
+        flx[x] = w.y;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- launcher
+static inline unsigned grid_for(int64_t tiles, int cap) {
+  return (unsigned)(tiles < cap ? (tiles > 0 ? tiles : 1) : cap);
+}
+
+#define LAUNCH_CHECK()                       \
+  do {                                       \
+    hipError_t e_ = hipGetLastError();       \
+    if (e_ != hipSuccess) return -(int)e_ - 1000; \
+  } while (0)
+
+int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream) {
+  if (P.tot_tab > 0) {
+    k_level_table<<<grid_for(P.tot_tab, grid_cap), 256, 0, stream>>>(P, B);
+    LAUNCH_CHECK();
+  }
+  const unsigned ft = (unsigned)P.fft_threads;
+  const size_t lds = (size_t)P.fft_lds_bytes;
+  if (P.any_dispersion) {
+    k_fwd_rows<<<grid_for(P.tot_fwd_rows, grid_cap), ft, lds, stream>>>(P, B);
+    LAUNCH_CHECK();
+    k_fwd_cols<<<grid_for(P.tot_fwd_cols, grid_cap), ft, lds, stream>>>(P, B);
+    LAUNCH_CHECK();
+  }
+  const unsigned gs = grid_for(P.tot_shoot, grid_cap);
+#define SHOOT(V, MAXS, RECOMP, EX)                                                    \
+  if (P.variant_used[V][EX]) {                                                        \
+    k_shoot<MAXS, RECOMP, EX><<<gs, SHOOT_THREADS, 0, stream>>>(P, B, V);             \
+    LAUNCH_CHECK();                                                                   \
+  }
+  SHOOT(0, 1, false, true)
+  SHOOT(1, 4, false, true)
+  SHOOT(2, 1, true, true)
+  SHOOT(0, 1, false, false)
+  SHOOT(1, 4, false, false)
+  SHOOT(2, 1, true, false)
+#undef SHOOT
+  if (P.variant_used[3][0] || P.variant_used[3][1]) {
+    k_shoot_analytic<<<gs, SHOOT_THREADS, 0, stream>>>(P, B, 3);
+    LAUNCH_CHECK();
+  }
+  if (P.tot_nyq > 0) {
+    k_nyq_fix<<<grid_for(P.tot_nyq, grid_cap), 256, 0, stream>>>(P, B);
+    LAUNCH_CHECK();
+  }
+  k_inv_cols<<<grid_for(P.tot_inv_cols, grid_cap), ft, lds, stream>>>(P, B);
+  LAUNCH_CHECK();
+  k_inv_rows<<<grid_for(P.tot_inv_rows, grid_cap), ft, lds, stream>>>(P, B);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+// ---------------------------------------------------------------- profiles
+// Array part of vertical_profiles (src/bldfm/pbl_model.py:127-182):
+// out = [z | u | v | Kx | Ky | Kz], nlev each.
+__global__ __launch_bounds__(256) void k_profiles(ProfSetup S, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.nlev) return;
+  double o[6];
+  prof_point(S, i, o);
+  for (int c = 0; c < 6; ++c) out[(int64_t)c * S.nlev + i] = o[c];
+}
+
+int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream) {
+  k_profiles<<<(S.nlev + 255) / 256, 256, 0, stream>>>(S, out);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+// ---------------------------------------------------------------- plain 2-D FFT
+// numpy fft2/ifft2 on `batch` contiguous (n0, n1) complex arrays: x-pass
+// over the rows into the workspace, then y-pass over the columns, output
+// multiplied by `scale`.
+__global__ void k_fft2_rows(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ in,
+                            double2* __restrict__ mid, int64_t nrows, bool inv) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const double2* src = in + r * F.n;
+    for (int i = tid; i < F.n; i += T) smem[i] = src[i];
+    __syncthreads();
+    bldfm_fft::lds_fft(smem, F, tw, inv, tid, T);
+    double2* dst = mid + r * F.n;
+    for (int i = tid; i < F.n; i += T) dst[i] = smem[i];
+    __syncthreads();
+  }
+}
+
+__global__ void k_fft2_cols(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ mid,
+                            double2* __restrict__ out, int32_t n1, int64_t ncols, double scale,
+                            bool inv) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int n0 = F.n;
+  for (int64_t c = blockIdx.x; c < ncols; c += gridDim.x) {
+    const int64_t b = c / n1, col = c % n1;
+    const double2* src = mid + b * (int64_t)n0 * n1 + col;
+    for (int i = tid; i < n0; i += T) smem[i] = src[(int64_t)i * n1];
+    __syncthreads();
+    bldfm_fft::lds_fft(smem, F, tw, inv, tid, T);
+    double2* dst = out + b * (int64_t)n0 * n1 + col;
+    for (int i = tid; i < n0; i += T) dst[(int64_t)i * n1] = cscale(scale, smem[i]);
+    __syncthreads();
+  }
+}
+
+int bldfm_launch_fft2(const FftPlan& F0, const double2* tw0, const FftPlan& F1, const double2* tw1,
+                      int32_t batch, const double2* in, double2* mid, double2* out, int32_t sign,
+                      double scale, int threads, int lds_bytes, int grid_cap, hipStream_t stream) {
+  const bool inv = sign > 0;
+  const int64_t nrows = (int64_t)batch * F0.n, ncols = (int64_t)batch * F1.n;
+  k_fft2_rows<<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, nrows, inv);
+  LAUNCH_CHECK();
+  k_fft2_cols<<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, F1.n, ncols,
+                                                                          scale, inv);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+// Allow dynamic LDS beyond the default per-workgroup cap for the FFT kernels.
+void bldfm_set_fft_lds_attr(int bytes) {
+  const void* fns[] = {(const void*)k_fwd_rows, (const void*)k_fwd_cols, (const void*)k_inv_cols,
+                       (const void*)k_inv_rows, (const void*)k_fft2_rows, (const void*)k_fft2_cols};
+  for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// ---------------------------------------------------------------- elementwise
+// psi(x), phi(x) of MOST (src/bldfm/pbl_model.py:207-250).
+__global__ __launch_bounds__(256) void k_stability(const double* __restrict__ x, int64_t n,
+                                                   double* __restrict__ psi, double* __restrict__ phi) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[i];
+    if (psi) psi[i] = most_psi(v);
+    if (phi) phi[i] = most_phi(v);
+  }
+}
+
+// Met wind (speed, direction blowing FROM, degrees) -> (u, v)
+// (src/bldfm/utils.py:23-27): u = -U sin(theta), v = -U cos(theta).
+__global__ __launch_bounds__(256) void k_wind(const double* __restrict__ spd, const double* __restrict__ dir,
+                                              int64_t n, double* __restrict__ u, double* __restrict__ v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double th = dir[i] * (BLDFM_PI / 180.0);
+    double s, c;
+    sincos(th, &s, &c);
+    u[i] = -spd[i] * s;
+    v[i] = -spd[i] * c;
+  }
+}
+
+int bldfm_launch_stability(const double* x, int64_t n, double* psi, double* phi, hipStream_t stream) {
+  if (n <= 0) return 0;
+  k_stability<<<grid_for((n + 255) / 256, 4096), 256, 0, stream>>>(x, n, psi, phi);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int bldfm_launch_wind(const double* spd, const double* dir, int64_t n, double* u, double* v, hipStream_t stream) {
+  if (n <= 0) return 0;
+  k_wind<<<grid_for((n + 255) / 256, 4096), 256, 0, stream>>>(spd, dir, n, u, v);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+// ---------------------------------------------------------------- math probe
+// Evaluates the device math helpers on n inputs for bitwise checks:
+// op 0 hypot_borges(a, b); op 1 csqrt_exact(a + ib) -> (out[2i], out[2i+1]);
+// op 2 cdiv((a0 + i a1), (b0 + i b1)) with a, b holding complex pairs;
+// op 3 ncmul of complex pairs; op 4 xcmul of complex pairs; op 5 sqrt(a).
+__global__ __launch_bounds__(256) void k_math_probe(int op, const double* __restrict__ a,
+                                                    const double* __restrict__ b, int64_t n,
+                                                    double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (op == 0) {
+      out[i] = hypot_borges(a[i], b[i]);
+    } else if (op == 1) {
+      const double2 r = csqrt_exact(cmk(a[i], b[i]));
+      out[2 * i] = r.x;
+      out[2 * i + 1] = r.y;
+    } else if (op == 5) {
+      out[i] = sqrt(a[i]);
+    } else {
+      const double2 x = cmk(a[2 * i], a[2 * i + 1]), y = cmk(b[2 * i], b[2 * i + 1]);
+      const double2 r = op == 2 ? cdiv(x, y) : (op == 3 ? ncmul(x, y) : xcmul(x, y));
+      out[2 * i] = r.x;
+      out[2 * i + 1] = r.y;
+    }
+  }
+}
+
+int bldfm_launch_math_probe(int op, const double* a, const double* b, int64_t n, double* out, hipStream_t stream) {
+  if (n <= 0) return 0;
+  k_math_probe<<<grid_for((n + 255) / 256, 4096), 256, 0, stream>>>(op, a, b, n, out);
+  LAUNCH_CHECK();
+  return 0;
+}

@@ -1,0 +1,231 @@
+"""Drop-in ``steady_state_transport_solver`` running on one MI355X.
+
+Same signature, returned arrays, dtypes and error messages as
+src/bldfm/solver.py:16-304.  The host side repeats the reference's argument
+handling (mode parity, halo, pad widths, mode clamping, level membership,
+output dtype), then one ``bldfm_plan_execute`` call runs the whole numeric
+path on the GPU: level tables, pruned forward DFT of the padded surface flux
+(dispersion mode), the per-mode vertical solve with shooting (FP64), the
+(0,0) trapezoid, the phase shift, the Hermitian packing of p^ and q^ and
+the pruned inverse DFT with the halo crop.
+
+``solve_device`` is the device-resident variant: torch tensors in, torch
+tensors out, no host copies (what the batched paths and the benchmark use).
+"""
+
+import collections
+import logging
+import threading
+
+import numpy as np
+import torch
+
+from . import _lib
+
+logger = logging.getLogger("bldfm_amd.solver")
+
+_PLAN_CACHE_SIZE = 32
+_plan_cache = collections.OrderedDict()
+_plan_lock = threading.Lock()
+
+
+def _level_maps(levels, nz):
+    """Slot map of the reference's ``i in levels`` test (ascending i) and
+    the per-slot level index used by the analytic branch (caller order)."""
+    lv = np.asarray(levels)
+    slot = np.full(nz, -1, dtype=np.int32)
+    k = 0
+    for i in range(nz):
+        if i in levels:
+            slot[i] = k
+            k += 1
+    idx = np.mod(lv.astype(np.int64), nz).astype(np.int32) if lv.dtype.kind in "iu" else np.zeros(len(lv), np.int32)
+    return slot, idx
+
+
+def _geometry(srf_flx_shape, z, domain, levels, modes, meas_pt, footprint, halo, precision):
+    """Reference argument handling, src/bldfm/solver.py:82-130, 177-188."""
+    xmx, ymx = domain
+    nlx, nly = modes
+    xm, ym = meas_pt
+    if (nlx % 2 > 0) or (nly % 2 > 0):
+        raise ValueError("modes must consist of even numbers.")
+    ny, nx = srf_flx_shape
+    nz = len(z)
+    dx, dy = xmx / nx, ymx / ny
+    if np.ndim(levels) == 0:
+        levels = np.array([levels])
+    if halo is None:
+        halo = max(xmx, ymx)
+    px, py = int(halo / dx), int(halo / dy)
+    nxe, nye = nx + 2 * px, ny + 2 * py
+    if (nlx > nxe) or (nly > nye):
+        logger.info("Warning: Number of Fourier modes must not exeed number of grid cells.")
+        logger.info("Setting both equal.")
+        nlx, nly = nxe, nye
+    if precision not in ("single", "double"):
+        raise ValueError("precision must be single (default) or double.")
+    if (nxe - nlx) % 2 or (nye - nly) % 2:
+        raise ValueError(
+            "padded grid and mode counts must have equal parity "
+            f"(padded {nye}x{nxe}, modes {nly}x{nlx})."
+        )
+    if footprint:
+        shift = (xm + halo, ym + halo)
+        has_shift = True
+    elif xm**2 + ym**2 > 0.0:
+        shift = (xm - xmx / 2, ym - ymx / 2)
+        has_shift = True
+    else:
+        shift, has_shift = (0.0, 0.0), False
+    return dict(
+        nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, px=px, py=py, nlx=nlx, nly=nly,
+        levels=levels, halo=halo, shift=shift, has_shift=has_shift,
+        out_f32=(precision == "single" and not has_shift),
+    )
+
+
+def _instance(g, nslot, footprint, analytic, precision, srf_bg_conc, fast=False):
+    inst = _lib.Instance()
+    inst.nx, inst.ny = g["nx"], g["ny"]
+    inst.nlx, inst.nly = g["nlx"], g["nly"]
+    inst.px, inst.py = g["px"], g["py"]
+    inst.nz = g["nz"]
+    inst.nslot = nslot
+    inst.footprint = bool(footprint)
+    inst.analytic = bool(analytic)
+    inst.single = precision == "single"
+    inst.out_f32 = g["out_f32"]
+    inst.has_shift = g["has_shift"]
+    inst.prof_from_met = 0
+    inst.fast = bool(fast)
+    inst.dx, inst.dy = g["dx"], g["dy"]
+    inst.shift_x, inst.shift_y = float(g["shift"][0]), float(g["shift"][1])
+    inst.srf_bg_conc = float(np.real(srf_bg_conc))
+    return inst
+
+
+def _get_plan(inst, ints, dev):
+    key = (dev.index, bytes(inst), ints.tobytes())
+    with _plan_lock:
+        plan = _plan_cache.get(key)
+        if plan is not None:
+            _plan_cache.move_to_end(key)
+            return plan
+    plan = _lib.Plan([inst], ints, dev)
+    with _plan_lock:
+        _plan_cache[key] = plan
+        while len(_plan_cache) > _PLAN_CACHE_SIZE:
+            _plan_cache.popitem(last=False)
+    return plan
+
+
+def solve_device(
+    srf_flx,
+    z,
+    profiles,
+    domain,
+    levels,
+    modes=(512, 512),
+    meas_pt=(0.0, 0.0),
+    srf_bg_conc=0.0,
+    footprint=False,
+    analytic=False,
+    halo=None,
+    precision="single",
+    fast=False,
+):
+    """Device-resident solve: returns ``(conc, flx)`` torch tensors of shape
+    (len(levels), ny, nx) on the current HIP device (float64, or float32 for
+    single precision without phase shift), plus the geometry dict.
+
+    ``fast=True`` uses fused multiply-adds in the vertical march; the default
+    reproduces the reference's rounding step by step, which the shooting
+    cancellation requires for agreement beyond ~1e-7 in stable cases."""
+    shape = tuple(srf_flx.shape)
+    z_np = z.detach().cpu().numpy() if torch.is_tensor(z) else np.asarray(z, dtype=np.float64)
+    g = _geometry(shape, z_np, domain, levels, modes, meas_pt, footprint, halo, precision)
+    dev = _lib.device()
+    nz, nslot = g["nz"], len(g["levels"])
+    if nz < 2:
+        raise ValueError("the vertical grid needs at least two points")
+    if analytic and nslot > 1:
+        # src/bldfm/solver.py:197-201 broadcasts h (nlvls,) against the modes
+        raise ValueError(
+            "operands could not be broadcast together with shapes "
+            f"({g['nly'] * g['nlx'] - 1},) ({nslot},)"
+        )
+    slot, idx = _level_maps(g["levels"], nz)
+    ints = np.concatenate([slot, idx]).astype(np.int32)
+    inst = _instance(g, nslot, footprint, analytic, precision, srf_bg_conc, fast)
+    plan = _get_plan(inst, ints, dev)
+
+    def as_dev(a):
+        t = a if torch.is_tensor(a) else torch.as_tensor(np.asarray(a, dtype=np.float64))
+        return t.to(device=dev, dtype=torch.float64).reshape(-1)
+
+    prof = torch.cat([as_dev(z)] + [as_dev(p) for p in profiles])
+    if prof.numel() != 6 * nz:
+        raise ValueError("profiles must be five arrays of len(z)")
+    src = None
+    if not footprint:
+        src = srf_flx if torch.is_tensor(srf_flx) else torch.as_tensor(np.ascontiguousarray(srf_flx, dtype=np.float64))
+        src = src.to(device=dev, dtype=torch.float64).contiguous()
+    odt = torch.float32 if g["out_f32"] else torch.float64
+    conc = torch.empty((nslot, g["ny"], g["nx"]), dtype=odt, device=dev)
+    flx = torch.empty_like(conc)
+    ws = torch.empty(max(plan.workspace_bytes, 1), dtype=torch.uint8, device=dev)
+    plan.execute(prof, src, conc, flx, ws)
+    return conc, flx, g
+
+
+def steady_state_transport_solver(
+    srf_flx,
+    z,
+    profiles,
+    domain,
+    levels,
+    modes=(512, 512),
+    meas_pt=(0.0, 0.0),
+    srf_bg_conc=0.0,
+    footprint=False,
+    analytic=False,
+    halo=None,
+    precision="single",
+    cache=None,
+):
+    """Steady-state advection-diffusion solve (src/bldfm/solver.py:16-74).
+
+    Parameters and returns as in the reference: ``srf_flx`` (ny, nx) surface
+    kinematic flux, ``z`` vertical grid, ``profiles`` = (u, v, Kx, Ky, Kz),
+    ``domain`` = (xmax, ymax), ``levels`` output level index or indices,
+    ``modes`` = (nlx, nly), ``meas_pt`` = (xm, ym); returns
+    ``((X, Y, Z), conc, flx)`` squeezed, with conc/flx (len(levels), ny, nx)
+    of concentration and vertical flux (or Green's function and footprint
+    when ``footprint``).  ``cache`` is any object with the reference's
+    ``get``/``put`` protocol (src/bldfm/solver.py:76-80, 300-302).
+    """
+    if cache is not None and footprint:
+        cached = cache.get(z, profiles, domain, modes, meas_pt, halo, precision)
+        if cached is not None:
+            return cached
+    q0 = np.asarray(srf_flx)
+    if q0.ndim != 2:
+        raise ValueError("srf_flx must be a 2-D array")
+    z = np.asarray(z, dtype=np.float64)
+    xmx, ymx = domain
+    lv = np.array([levels]) if np.ndim(levels) == 0 else levels
+    zl = z[lv]  # the reference indexes z by levels (IndexError for bad levels)
+    conc_t, flx_t, g = solve_device(
+        q0, z, profiles, domain, lv, modes, meas_pt, srf_bg_conc, footprint, analytic, halo, precision
+    )
+    conc = conc_t.cpu().numpy()
+    flx = flx_t.cpu().numpy()
+    ny, nx = q0.shape
+    x = np.linspace(0, xmx, nx, endpoint=False)
+    y = np.linspace(0, ymx, ny, endpoint=False)
+    Zg, Yg, Xg = np.meshgrid(zl, y, x, indexing="ij")
+    result = ((np.squeeze(Xg), np.squeeze(Yg), np.squeeze(Zg)), np.squeeze(conc), np.squeeze(flx))
+    if cache is not None and footprint:
+        cache.put(z, profiles, domain, modes, meas_pt, halo, precision, *result)
+    return result

@@ -1,0 +1,185 @@
+/*
+ * bldfm_amd.h -- C ABI of the MI355X-native BLDFM steady-state solver.
+ *
+ * The reference (SchlutowSM2Group/BLDFM) is pure Python; its hot path is
+ *   steady_state_transport_solver   src/bldfm/solver.py:16-304
+ *   ivp_solver (numba JIT)          src/bldfm/solver.py:307-374
+ *   vertical_profiles / psi / phi   src/bldfm/pbl_model.py:8-250
+ *   FFT backend (pyFFTW -> FFTW3)   src/bldfm/fft_manager.py:78-157
+ * The Python drop-ins in bldfm_amd/ keep those signatures and call the
+ * functions below through ctypes.  No torch types cross this boundary:
+ * every buffer is a plain device pointer (the Python layer allocates them
+ * as torch tensors) and every stream is a hipStream_t.
+ *
+ * Conventions: functions return 0 on success and a negative code on error;
+ * bldfm_last_error() returns the calling thread's last message.  Nothing
+ * here frees or retains caller memory.  A plan may be executed on any
+ * stream of the device it was created on; executions of one plan that
+ * overlap in time must use distinct workspaces.
+ */
+#ifndef BLDFM_AMD_H
+#define BLDFM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLDFM_ABI_VERSION 1
+
+/* Error codes */
+#define BLDFM_OK 0
+#define BLDFM_EINVAL -1   /* invalid argument (message says which) */
+#define BLDFM_ENOMEM -2   /* device allocation failed */
+#define BLDFM_EHIP -3     /* HIP runtime error */
+#define BLDFM_ELIMIT -4   /* size beyond what the kernels support */
+
+/* Closure ids (src/bldfm/pbl_model.py:65-182). */
+#define BLDFM_CLOSURE_MOST 0
+#define BLDFM_CLOSURE_CONSTANT 1
+#define BLDFM_CLOSURE_MOSTM 2
+#define BLDFM_CLOSURE_OAAHOC 3
+
+/*
+ * Meteorology / profile parameters: the arguments of
+ * vertical_profiles(n, meas_height, wind, ustar, z0, mol, prsc, closure,
+ * domain_height, stretch, z0_min, z0_max, tke)   src/bldfm/pbl_model.py:8-22.
+ * Exactly one of has_ustar/has_z0 for MOST, MOSTM and CONSTANT.
+ */
+typedef struct bldfm_met {
+  int32_t n;                 /* grid points between z0 and meas_height */
+  int32_t closure;           /* BLDFM_CLOSURE_* */
+  int32_t has_ustar, has_z0, has_domain_height, has_stretch;
+  double meas_height;        /* zm [m] */
+  double um, vm;             /* wind components at zm [m/s] */
+  double ustar, z0, mol, prsc, domain_height, stretch, tke;
+} bldfm_met;
+
+/*
+ * One solve: the arguments of steady_state_transport_solver after the
+ * reference's validation and halo/mode arithmetic (src/bldfm/solver.py:82-130).
+ */
+typedef struct bldfm_instance {
+  int32_t nx, ny;            /* srf_flx is (ny, nx), C order */
+  int32_t nlx, nly;          /* modes after clamping to the padded grid */
+  int32_t px, py;            /* halo pad widths int(halo/dx), int(halo/dy) */
+  int32_t nz;                /* len(z); ignored when prof_from_met (derived) */
+  int32_t nslot;             /* len(levels) */
+  int32_t footprint;         /* 1: Green's function / footprint */
+  int32_t analytic;          /* 1: constant-profile closed form */
+  int32_t single;            /* 1: precision="single" (complex64 spectra) */
+  int32_t out_f32;           /* 1: write float32 outputs (single, no shift) */
+  int32_t has_shift;         /* 1: apply the phase shift below */
+  int32_t prof_from_met;     /* 1: evaluate profiles from met on the device */
+  int32_t fast;              /* 0: reproduce the reference's rounding in the vertical
+                                march (default); 1: fused multiply-adds (faster,
+                                agrees to the roundoff of the shooting cancellation) */
+  int32_t reserved;          /* must be 0 */
+  double dx, dy;             /* xmax/nx, ymax/ny */
+  double shift_x, shift_y;   /* xm+halo, ym+halo (footprint) or xm-xmax/2, ym-ymax/2 */
+  double srf_bg_conc;        /* p000 */
+  bldfm_met met;             /* used when prof_from_met */
+} bldfm_instance;
+
+typedef struct bldfm_plan bldfm_plan;
+
+/* Version of this ABI (BLDFM_ABI_VERSION). */
+int bldfm_abi_version(void);
+
+/* Last error message of the calling thread. */
+const char* bldfm_last_error(void);
+
+/*
+ * Replaces the scalar prologue of vertical_profiles
+ * (src/bldfm/pbl_model.py:58-129): number of grid points and the derived
+ * roughness length / friction velocity.
+ */
+int bldfm_profile_query(const bldfm_met* met, int32_t* nlev, double* z0, double* ustar);
+
+/*
+ * Replaces the array part of vertical_profiles (src/bldfm/pbl_model.py:127-182):
+ * writes z,u,v,Kx,Ky,Kz (6*nlev doubles, each array contiguous) to prof6_dev
+ * on the device.
+ */
+int bldfm_profiles(const bldfm_met* met, double* prof6_dev, hipStream_t stream);
+
+/*
+ * MOST stability functions on n device values x = z/L: psi (momentum
+ * correction) and phi (diffusivity), either output may be null.
+ * Replaces psi / phi (src/bldfm/pbl_model.py:207-250).
+ */
+int bldfm_stability(const double* x, int64_t n, double* psi, double* phi, hipStream_t stream);
+
+/*
+ * Met wind speed and direction (degrees, blowing FROM, clockwise from north)
+ * to (u, v) for n device values.  Replaces compute_wind_fields
+ * (src/bldfm/utils.py:7-27).
+ */
+int bldfm_wind_fields(const double* speed, const double* dir, int64_t n, double* u, double* v,
+                      hipStream_t stream);
+
+/*
+ * Builds the device metadata (descriptors, tile tables, FFT twiddles) for a
+ * batch of ninst solves on `device`.  `levels` holds, per instance in order,
+ * nz level->slot entries (slot index or -1; the reference's `i in levels`
+ * membership in ascending i, src/bldfm/solver.py:244,352) followed by nslot
+ * level indices in the caller's order (used by the analytic branch,
+ * src/bldfm/solver.py:197).  For prof_from_met instances nz is the value
+ * bldfm_profile_query returns.  All instances must share out_f32.
+ * Replaces the per-call setup of steady_state_transport_solver
+ * (src/bldfm/solver.py:82-191) and the FFTW plan cache
+ * (src/bldfm/fft_manager.py:23-44).
+ */
+int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst,
+                      const int32_t* levels, bldfm_plan** plan);
+
+int bldfm_plan_destroy(bldfm_plan* plan);
+
+/* Buffer sizes of a plan: workspace bytes, profile doubles (6*nz summed),
+ * source doubles (ny*nx summed over dispersion instances), output elements
+ * (nslot*ny*nx summed). */
+int bldfm_plan_sizes(const bldfm_plan* plan, size_t* workspace_bytes, int64_t* prof_doubles,
+                     int64_t* src_doubles, int64_t* out_elems);
+
+/* Element offsets of instance i inside the prof, src and output buffers. */
+int bldfm_plan_offsets(const bldfm_plan* plan, int32_t i, int64_t* off_prof, int64_t* off_src,
+                       int64_t* off_out, int32_t* nz);
+
+/*
+ * Runs the whole path on `stream`: level tables (and profiles for
+ * prof_from_met instances), forward transform of srf_flx (dispersion
+ * instances), vertical solve with shooting, inverse transforms, crop.
+ * Replaces src/bldfm/solver.py:132-290 including ivp_solver and the two
+ * fft2/ifft2 calls.  conc/flx receive (nslot, ny, nx) per instance as float64,
+ * or float32 when out_f32.  Asynchronous; capturable into a hipGraph.
+ */
+int bldfm_plan_execute(bldfm_plan* plan, double* prof, const double* src, void* conc, void* flx,
+                       void* workspace, hipStream_t stream);
+
+/*
+ * 2-D complex FFT of `batch` contiguous (n0, n1) complex128 arrays with
+ * numpy semantics: sign -1 (fft2) or +1 (ifft2), every output multiplied by
+ * `scale`.  Replaces FFTManager.fft2/ifft2 (src/bldfm/fft_manager.py:78-102).
+ * `workspace` needs bldfm_fft2_workspace_bytes().
+ */
+size_t bldfm_fft2_workspace_bytes(int32_t n0, int32_t n1, int32_t batch);
+int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* in, double* out,
+               int32_t sign, double scale, void* workspace, hipStream_t stream);
+
+/*
+ * Diagnostics (no counterpart in the reference): evaluates the device
+ * arithmetic helpers on n device inputs so tests can compare their rounding
+ * with the host's numpy/libm bit for bit.  op: 0 hypot(a,b); 1 complex sqrt
+ * of a+ib (2n outputs); 2 complex divide, 3 numpy-rounded complex multiply,
+ * 4 plain complex multiply (a, b hold n complex pairs, 2n outputs); 5 sqrt(a).
+ */
+int bldfm_math_probe(int32_t op, const double* a, const double* b, int64_t n, double* out,
+                     hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLDFM_AMD_H */

@@ -1,0 +1,245 @@
+"""GPU parity: the HIP path against the reference fixtures and the oracle.
+
+Bars (stated per test):
+* reference regression fixtures (generated with precision="single"):
+  atol 1e-6 / rtol 1e-5 as in the reference's tests/test_regression.py:17-24,
+  and rel-L2 <= 1e-6 against the oracle run the same way;
+* precision="double": rel-L2 <= 1e-10 against the FP64 oracle on identical
+  inputs (profiles from the oracle for both, so the solver is compared alone);
+* analytic branch: rel-L2 <= 1e-10 against the oracle's closed form; the
+  numerical solve within 1e-3 of the analytic maximum
+  (tests/test_integration.py:28-84 of the reference).
+"""
+
+import numpy as np
+import pytest
+
+import scenarios
+from conftest import load_golden, rel_l2
+from oracle import pbl as opbl
+from oracle import solver as osolver
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from bldfm_amd import _lib, pbl_model, solver
+
+    _lib.load()
+    return pbl_model, solver
+
+
+FIXTURES = ["source_area", "plume_3d", "single_footprint", "timeseries_step2"]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_reference_fixture(gpu, name):
+    pbl_model, solver = gpu
+    print(f"[fixture] {name}", flush=True)
+    grid, conc, flx = scenarios.run(name, pbl_model.vertical_profiles, solver.steady_state_transport_solver)
+    ref = load_golden(name)
+    assert conc.shape == ref["conc"].shape and flx.shape == ref["flx"].shape
+    np.testing.assert_allclose(conc, ref["conc"], atol=1e-6, rtol=1e-5)
+    np.testing.assert_allclose(flx, ref["flx"], atol=1e-6, rtol=1e-5)
+    for k, g in zip("XYZ", grid):
+        np.testing.assert_array_equal(g, ref[k])
+    _, oc, of = scenarios.run(name, opbl.vertical_profiles, osolver.steady_state_transport_solver)
+    assert conc.dtype == oc.dtype
+    assert rel_l2(conc, oc) <= 1e-6 and rel_l2(flx, of) <= 1e-6
+
+
+DOUBLE_CASES = ["source_area", "plume_3d", "single_footprint", "c1_dispersion", "c1_footprint",
+                "c2_unstable_footprint", "c2_stable_footprint"]
+
+
+@pytest.mark.parametrize("name", DOUBLE_CASES)
+def test_double_parity(gpu, name):
+    _, solver = gpu
+    print(f"[double] {name}", flush=True)
+    q0, z, profs, domain, levels, kw = scenarios.inputs(name, opbl.vertical_profiles)
+    kw["precision"] = "double"
+    _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    assert gc.dtype == np.float64
+    ec, ef = rel_l2(gc, oc), rel_l2(gf, of)
+    print(f"  rel-L2 conc {ec:.3e} flx {ef:.3e}")
+    assert ec <= 1e-10 and ef <= 1e-10
+
+
+def test_footprint_integral(gpu):
+    """Footprint integral (sum of flx over the crop) matches the oracle."""
+    _, solver = gpu
+    q0, z, profs, domain, levels, kw = scenarios.inputs("c1_footprint", opbl.vertical_profiles)
+    _, _, gf = solver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    _, _, of = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    assert abs(gf.sum() - of.sum()) <= 1e-12 * abs(of.sum())
+
+
+@pytest.mark.parametrize("footprint", [False, True])
+def test_analytic_branch(gpu, footprint):
+    _, solver = gpu
+    nxy, domain = (128, 64), (100.0, 50.0)
+    q0 = opbl.ideal_source(nxy, domain, (5.0, 5.0), shape="point")
+    z, profs = opbl.vertical_profiles(16, 5.0, (4.0, 1.0), 0.2, closure="CONSTANT")
+    kw = dict(modes=(128, 128), halo=500.0, analytic=True, precision="double", footprint=footprint,
+              meas_pt=(50.0, 25.0) if footprint else (0.0, 0.0))
+    for level in (16, 9):
+        _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, domain, level, **kw)
+        _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, level, **kw)
+        assert rel_l2(gc, oc) <= 1e-10 and rel_l2(gf, of) <= 1e-10
+    # several levels fail in the reference (h (nlvls,) broadcast against the modes)
+    with pytest.raises(ValueError, match="broadcast"):
+        osolver.steady_state_transport_solver(q0, z, profs, domain, [3, 16], **kw)
+    with pytest.raises(ValueError, match="broadcast"):
+        solver.steady_state_transport_solver(q0, z, profs, domain, [3, 16], **kw)
+
+
+@pytest.mark.parametrize("name", ["c1_footprint", "c2_unstable_footprint", "c2_stable_footprint"])
+def test_fast_mode_error(gpu, name):
+    """The fused-multiply-add march agrees to the shooting roundoff floor."""
+    _, solver = gpu
+    q0, z, profs, domain, levels, kw = scenarios.inputs(name, opbl.vertical_profiles)
+    conc, flx, _ = solver.solve_device(q0, z, profs, domain, levels, fast=True, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    ec, ef = rel_l2(conc.cpu().numpy()[0], oc), rel_l2(flx.cpu().numpy()[0], of)
+    print(f"  fast-mode rel-L2 conc {ec:.3e} flx {ef:.3e}")
+    assert ec <= 1e-6 and ef <= 1e-6
+
+
+def test_numerical_vs_analytic_reference_size(gpu):
+    """tests/test_integration.py:28-84 at its own size: 256^2 grid, 512^2
+    modes, nz=256, halo 500 (padded 2816 x 5376)."""
+    _, solver = gpu
+    nxy, domain = (256, 256), (100.0, 50.0)
+    q0 = opbl.ideal_source(nxy, domain, (5.0, 5.0), shape="point")
+    z, profs = opbl.vertical_profiles(256, 5.0, (4.0, 1.0), 0.2, closure="CONSTANT")
+    _, ca, fa = solver.steady_state_transport_solver(q0, z, profs, domain, 256, modes=(512, 512), halo=500.0, analytic=True)
+    _, cn, fn = solver.steady_state_transport_solver(q0, z, profs, domain, 256, modes=(512, 512), halo=500.0)
+    dc = np.abs(cn - ca).max() / np.max(ca)
+    df = np.abs(fn - fa).max() / np.max(fa)
+    print(f"  max err conc {dc:.3e} flx {df:.3e}")
+    assert dc <= 1e-3 and df <= 1e-3
+
+
+def test_levels_semantics(gpu):
+    """Duplicate, unsorted and out-of-column levels follow `i in levels`."""
+    _, solver = gpu
+    q0, z, profs, domain, _, kw = scenarios.inputs("plume_3d", opbl.vertical_profiles)
+    kw["precision"] = "double"
+    for levels in ([5, 2, 2], [0], [len(z) - 1, 3], [1, 2, 3, 4, 5, 6, 7]):
+        _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+        _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+        assert gc.shape == oc.shape
+        assert rel_l2(gc, oc) <= 1e-10 and rel_l2(gf, of) <= 1e-10, levels
+
+
+def test_mode_clamp_and_no_halo(gpu):
+    """modes beyond the padded grid are clamped (src/bldfm/solver.py:122-127)."""
+    _, solver = gpu
+    q0 = opbl.ideal_source((128, 64), (500.0, 250.0), (250.0, 125.0), shape="point")
+    z, profs = opbl.vertical_profiles(16, 10.0, (5.0, 0.0), 0.4)
+    kw = dict(modes=(512, 512), halo=1.0, precision="double")
+    _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, (500.0, 250.0), 15, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, (500.0, 250.0), 15, **kw)
+    assert np.isfinite(gf).all()
+    assert rel_l2(gc, oc) <= 1e-10 and rel_l2(gf, of) <= 1e-10
+
+
+def test_single_precision_dtype(gpu):
+    _, solver = gpu
+    q0, z, profs, domain, levels, kw = scenarios.inputs("c1_dispersion", opbl.vertical_profiles)
+    kw["precision"] = "single"
+    _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    assert gc.dtype == np.float32 == oc.dtype
+    assert rel_l2(gc, oc) <= 1e-6 and rel_l2(gf, of) <= 1e-6
+
+
+@pytest.mark.parametrize(
+    "kw",
+    [
+        dict(n=16, meas_height=10.0, wind=(0.0, -6.0), ustar=0.5),
+        dict(n=10, meas_height=5.0, wind=(3.0, 1.0), ustar=0.4, mol=-100.0),
+        dict(n=20, meas_height=10.0, wind=(2.0, 2.0), ustar=0.3, mol=-50.0),
+        dict(n=26, meas_height=15.0, wind=(-3.5, 0.1), ustar=0.3, mol=100.0, domain_height=100.0),
+        dict(n=10, meas_height=5.0, wind=(3.0, 1.0), z0=0.1, mol=-100.0),
+        dict(n=10, meas_height=5.0, wind=(3.0, 1.0), ustar=0.4, closure="CONSTANT", prsc=0.8),
+        dict(n=10, meas_height=5.0, wind=(3.0, 1.0), ustar=0.4, closure="MOSTM", mol=-100.0),
+        dict(n=10, meas_height=5.0, wind=(3.0, 1.0), ustar=0.4, closure="OAAHOC", tke=2.0),
+        dict(n=10, meas_height=5.0, wind=(3.0, 1.0), ustar=0.4, stretch=15.0, domain_height=50.0),
+    ],
+)
+def test_vertical_profiles(gpu, kw):
+    pbl_model, _ = gpu
+    z, p = pbl_model.vertical_profiles(**kw)
+    oz, op = opbl.vertical_profiles(**kw)
+    assert len(z) == len(oz)
+    np.testing.assert_allclose(z, oz, rtol=1e-13, atol=0)
+    for a, b in zip(p, op):
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+    if kw.get("closure", "MOST") != "MOSTM":
+        assert p[2] is p[3] is p[4]
+
+
+def test_psi_phi_and_wind(gpu):
+    from bldfm_amd import pbl_model, utils
+
+    x = np.concatenate([np.linspace(-5, 2, 301), [0.0, 1e-9, -1e-9]])
+    np.testing.assert_allclose(pbl_model.psi(x), opbl.psi(x), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(pbl_model.phi(x), opbl.phi(x), rtol=1e-13, atol=1e-15)
+    for spd, wd in [(5.0, 270.0), (6.0, 0.0), (7.0, 345.0), (3.0, 90.0), (2.0, 180.0)]:
+        u, v = utils.compute_wind_fields(spd, wd)
+        ou, ov = opbl.compute_wind_fields(spd, wd)
+        assert abs(u - ou) <= 1e-15 * spd and abs(v - ov) <= 1e-15 * spd
+
+
+@pytest.mark.parametrize("shape", [(8, 6), (384, 960), (3, 2, 544, 192), (17, 11)])
+@pytest.mark.parametrize("norm", ["backward", "forward", "ortho"])
+def test_fft2(gpu, shape, norm):
+    from bldfm_amd import fft_manager
+
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    for ours, ref in ((fft_manager.fft2, np.fft.fft2), (fft_manager.ifft2, np.fft.ifft2)):
+        got = ours(x, norm=norm)
+        exp = ref(x, norm=norm)
+        assert rel_l2(got.real, exp.real) <= 1e-13 and rel_l2(got.imag, exp.imag) <= 1e-13
+
+
+def test_device_math_rounds_like_host(gpu):
+    """Device hypot / complex sqrt / divide / multiply round exactly like the
+    host's numpy on the value ranges the solver feeds them."""
+    import torch
+
+    from bldfm_amd import _lib
+
+    lib = _lib.load()
+    dev = _lib.device()
+    rng = np.random.default_rng(11)
+    n = 200000
+    re = 10 ** rng.uniform(-8, 2, n)
+    im = rng.choice([-1.0, 1.0], n) * 10 ** rng.uniform(-8, 2, n)
+
+    def probe(op, a, b, nout):
+        ta = torch.as_tensor(a, device=dev)
+        tb = torch.as_tensor(b, device=dev) if b is not None else None
+        out = torch.empty(nout, dtype=torch.float64, device=dev)
+        _lib.check(lib.bldfm_math_probe(op, _lib.ptr(ta), _lib.ptr(tb), n, _lib.ptr(out), _lib.stream_handle(dev)))
+        return out.cpu().numpy()
+
+    assert np.array_equal(probe(0, re, im, n), np.hypot(re, im))
+    s = probe(1, re, im, 2 * n)
+    ref = np.sqrt(re + 1j * im)
+    assert np.array_equal(s[0::2], ref.real) and np.array_equal(s[1::2], ref.imag)
+    assert np.array_equal(probe(5, re, None, n), np.sqrt(re))
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    pa = np.stack([a.real, a.imag], 1).ravel()
+    pb = np.stack([b.real, b.imag], 1).ravel()
+    for op, ref in ((2, a / b), (3, a * b)):
+        r = probe(op, pa, pb, 2 * n)
+        assert np.array_equal(r[0::2], ref.real) and np.array_equal(r[1::2], ref.imag), op
