@@ -1,0 +1,207 @@
+"""Benchmark: batched FP64 footprint solves at 512^2 modes x 64 vertical steps.
+
+Workload (one "step" = one pass of the hot path over one batch): B footprint
+solves of the production configuration -- domain 2000 x 2000 m, 512 x 512
+grid and Fourier modes (default halo, padded grid 1536^2), measurement height
+15 m, 26 points below zm and a column to 100 m (65 levels = 64 vertical
+steps), u* = 0.3 m/s, z0 = 0.05 m, Obukhov length alternating -20 m / +100 m,
+seeded wind directions, tower at the domain centre.  Each solve produces the
+footprint and the Green's function at zm (complex FP64 spectra, FP64 output).
+Inputs (met scalars) are resident on the GPU before the timed region; each
+step evaluates profiles, marches every mode, shoots, and transforms back.
+
+Prints one JSON line (the driver's contract) with the roofline of the
+dominant kernel (the shooting kernel, FP64 VALU bound) measured with HIP
+events on the launch stream, and a CPU baseline: the numpy oracle timed on
+a bounded sample of the same workload on one host core.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+os.environ.setdefault("OMP_NUM_THREADS", "1")
+os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+os.environ.setdefault("MKL_NUM_THREADS", "1")
+
+import numpy as np  # noqa: E402
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "footprints/s at 512²×64: batched FP64 solver vs surrogate at rel-L2≤1e-3"
+PEAK_FP64 = 78.6e12  # MI355X FP64 vector, FMA counted as 2 FLOP
+FLOP_PER_MODE_STEP = 80  # algorithmic FP64 FLOP per (mode, vertical step), both IVPs
+
+
+def psi_bd(x):
+    if x > 0:
+        return 5.0 * x
+    xi = (1.0 - 16.0 * x) ** 0.25
+    return -2 * np.log(0.5 * (1 + xi)) - np.log(0.5 * (1 + xi * xi)) + 2 * np.arctan(xi) - 0.5 * np.pi
+
+
+def workload(batch, seed=20261204):
+    """Footprint specs of the 512^2 x 64 production configuration."""
+    from bldfm_amd.batch import FootprintSpec
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(batch):
+        mol = -20.0 if i % 2 == 0 else 100.0
+        speed = 0.3 / 0.4 * (np.log(15.0 / 0.05) + psi_bd(15.0 / mol))  # |U(zm)| giving z0 = 0.05
+        wd = np.deg2rad(rng.uniform(0.0, 360.0))
+        specs.append(
+            FootprintSpec(
+                nx=512, ny=512, domain=(2000.0, 2000.0), modes=(512, 512), meas_pt=(1000.0, 1000.0),
+                n=26, meas_height=15.0, wind=(-speed * np.sin(wd), -speed * np.cos(wd)), ustar=0.3,
+                mol=mol, domain_height=100.0,
+            )
+        )
+    return specs
+
+
+def timed(batch_obj, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        batch_obj.run()
+    torch.cuda.synchronize()
+    batch_obj.plan.timing_begin(steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch_obj.run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases, n = batch_obj.plan.timing_end()
+    return t1 - t0, phases, n
+
+
+def cpu_baseline(specs, gpu_batch, nsample):
+    """Oracle (numpy restatement of the reference algorithm) on one core, on
+    the first ``nsample`` footprints of the workload; also returns the
+    largest rel-L2 between those GPU footprints and the oracle's."""
+    from oracle import pbl as opbl
+    from oracle import solver as osolver
+
+    worst = 0.0
+    t = 0.0
+    for i in range(nsample):
+        s = specs[i]
+        z, profs = gpu_batch.profiles(i)  # same inputs for both solvers
+        t0 = time.perf_counter()
+        _, oc, of = osolver.steady_state_transport_solver(
+            np.zeros((s.ny, s.nx)), z, profs, s.domain, s.n, modes=s.modes, meas_pt=s.meas_pt,
+            footprint=True, precision="double",
+        )
+        t += time.perf_counter() - t0
+        gc, gf = gpu_batch.output(i)
+        gc, gf = gc[0].cpu().numpy(), gf[0].cpu().numpy()
+        for a, b in ((gc, oc), (gf, of)):
+            worst = max(worst, float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+        # profiles: device evaluation vs the oracle's
+        oz, _ = opbl.vertical_profiles(s.n, s.meas_height, s.wind, ustar=s.ustar, mol=s.mol,
+                                       domain_height=s.domain_height)
+        assert len(oz) == len(z)
+    return nsample / t, worst
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--cpu-sample", type=int, default=3)
+    ap.add_argument("--no-fast", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(batches shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from bldfm_amd.batch import FootprintBatch
+
+    torch.cuda.set_device(0)
+    specs = workload(args.batch)
+    print(f"[bench] batch={args.batch} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    fb = FootprintBatch(specs, precision="double", fast=False)
+    elapsed, phases, n = timed(fb, args.steps, args.warmup)
+    fps = args.batch * args.steps / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+    units = sum(l * s for l, s in fb.lanes_and_steps())  # (mode, step) units per launch
+    shoot_ms = phases["shoot"] / max(n, 1)
+    achieved = FLOP_PER_MODE_STEP * units / (shoot_ms * 1e-3)
+    print(f"[bench] exact: {fps:.1f} footprints/s, {ms_step:.3f} ms/step, phases(ms/step)="
+          + ", ".join(f"{k}={v / max(n, 1):.3f}" for k, v in phases.items()), file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_fast:
+        fbf = FootprintBatch(specs, precision="double", fast=True)
+        el_f, ph_f, n_f = timed(fbf, args.steps, args.warmup)
+        extra["fast_fma_value"] = round(args.batch * args.steps / el_f, 2)
+        extra["fast_fma_shoot_ms"] = round(ph_f["shoot"] / max(n_f, 1), 4)
+        d = 0.0
+        for i in range(min(4, args.batch)):
+            for a, b in zip(fbf.output(i), fb.output(i)):
+                a, b = a.cpu().numpy(), b.cpu().numpy()
+                d = max(d, float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+        extra["fast_fma_rel_l2_vs_exact"] = d
+        del fbf
+
+    cpu_fps, worst = cpu_baseline(specs, fb, args.cpu_sample)
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "footprints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded met per footprint; delta-source footprints)",
+        "config": {
+            "workload": "batched FP64 footprint solver, 512x512 grid and modes, 2000 m domain, "
+                        "padded 1536^2, 65 levels (64 steps) to 101.5 m, zm=15 m, L=-20/+100 m",
+            "footprints_per_step": args.batch,
+            "rounding": "reference (numba/numpy) rounding reproduced; no fused multiply-add in the march",
+        },
+        "roofline": {
+            "bound": "fp64-valu",
+            "kernel": "k_shoot<1,false,true> (vertical march + shooting)",
+            "achieved": round(achieved / 1e12, 3),
+            "peak": PEAK_FP64 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64, 4),
+            "traffic": None,
+            "flop_per_unit": FLOP_PER_MODE_STEP,
+            "units_per_launch": units,
+            "avg_launch_ms": round(shoot_ms, 4),
+            "issue_ceiling_frac_without_fma": 0.5,
+        },
+        "phase_ms_per_step": {k: round(v / max(n, 1), 4) for k, v in phases.items()},
+        "cpu_baseline": {
+            "value": round(cpu_fps, 4),
+            "unit": "footprints/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_sample} footprints of the same workload (numpy oracle, one core)",
+        },
+        "parity": {"max_rel_l2_vs_oracle": worst, "footprints_checked": args.cpu_sample},
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

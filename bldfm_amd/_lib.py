@@ -84,6 +84,8 @@ SIGNATURES = {
     "bldfm_plan_sizes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "bldfm_plan_offsets": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
     "bldfm_plan_execute": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "bldfm_plan_timing_begin": (ctypes.c_int, [_P, _I32]),
+    "bldfm_plan_timing_end": (ctypes.c_int, [_P, ctypes.POINTER(_D), ctypes.POINTER(_I32)]),
     "bldfm_stability": (ctypes.c_int, [_P, _I64, _P, _P, _P]),
     "bldfm_wind_fields": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     "bldfm_math_probe": (ctypes.c_int, [_I32, _P, _P, _I64, _P, _P]),
@@ -203,6 +205,18 @@ class Plan:
             lib.bldfm_plan_execute(self.handle, ptr(prof), ptr(src), ptr(conc), ptr(flx), ptr(workspace), s),
             "plan_execute",
         )
+
+    PHASES = ("level_table", "forward_fft", "shoot", "nyquist_fix", "inverse_y", "inverse_x")
+
+    def timing_begin(self, max_execs):
+        check(load().bldfm_plan_timing_begin(self.handle, int(max_execs)), "timing_begin")
+
+    def timing_end(self):
+        """Summed milliseconds per phase over the timed executions, and their count."""
+        ms = (ctypes.c_double * 6)()
+        n = _I32()
+        check(load().bldfm_plan_timing_end(self.handle, ms, ctypes.byref(n)), "timing_end")
+        return dict(zip(self.PHASES, list(ms))), n.value
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -151,7 +151,16 @@ struct bldfm_plan {
   PlanDev dev{};
   size_t ws_bytes = 0;
   int64_t tab_doubles = 0, ws_cplx = 0, prof_doubles = 0, src_doubles = 0, out_elems = 0;
+  // phase timing (bldfm_plan_timing_begin/end): 7 events per execution
+  std::vector<hipEvent_t> events;
+  int timing_max = 0, timing_count = 0;
 };
+
+static void destroy_events(bldfm_plan* P) {
+  for (hipEvent_t e : P->events) (void)hipEventDestroy(e);
+  P->events.clear();
+  P->timing_max = P->timing_count = 0;
+}
 
 extern "C" {
 
@@ -411,9 +420,10 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
 
 int bldfm_plan_destroy(bldfm_plan* plan) {
   if (!plan) return 0;
-  if (plan->dmeta) {
+  if (plan->dmeta || !plan->events.empty()) {
     DeviceGuard g(plan->device);
-    (void)hipFree(plan->dmeta);
+    destroy_events(plan);
+    if (plan->dmeta) (void)hipFree(plan->dmeta);
   }
   delete plan;
   return 0;
@@ -453,10 +463,49 @@ int bldfm_plan_execute(bldfm_plan* P, double* prof, const double* src, void* con
   B.flx = flx;
   B.tab = (double*)workspace;
   B.ws = (double2*)((char*)workspace + align_up(P->tab_doubles * 8, 256));
-  const int rc = bldfm_launch_all(P->dev, B, kGridCap, stream);
+  hipEvent_t* ev = nullptr;
+  if (P->timing_count < P->timing_max) ev = P->events.data() + 7 * (P->timing_count++);
+  const int rc = bldfm_launch_all(P->dev, B, kGridCap, stream, ev);
+  if (rc == -1999) return fail(BLDFM_EHIP, "hipEventRecord failed");
   if (rc) return fail(BLDFM_EHIP, std::string("kernel launch failed: ") +
                                       hipGetErrorString((hipError_t)(-(rc + 1000))));
   return 0;
+}
+
+int bldfm_plan_timing_begin(bldfm_plan* P, int32_t max_execs) {
+  if (!P || max_execs <= 0 || max_execs > 100000) return fail(BLDFM_EINVAL, "bad timing arguments");
+  DeviceGuard guard(P->device);
+  destroy_events(P);
+  P->events.resize((size_t)7 * max_execs, nullptr);
+  for (auto& e : P->events) {
+    hipError_t err = hipEventCreate(&e);
+    if (err != hipSuccess) { destroy_events(P); return hip_fail(err, "hipEventCreate"); }
+  }
+  P->timing_max = max_execs;
+  P->timing_count = 0;
+  return 0;
+}
+
+int bldfm_plan_timing_end(bldfm_plan* P, double* phase_ms, int32_t* nexecs) {
+  if (!P || !phase_ms) return fail(BLDFM_EINVAL, "bad timing arguments");
+  DeviceGuard guard(P->device);
+  for (int k = 0; k < 6; ++k) phase_ms[k] = 0.0;
+  const int n = P->timing_count;
+  int rc = 0;
+  if (n > 0) {
+    hipError_t err = hipEventSynchronize(P->events[7 * (n - 1) + 6]);
+    if (err != hipSuccess) rc = hip_fail(err, "hipEventSynchronize");
+    for (int x = 0; x < n && !rc; ++x)
+      for (int k = 0; k < 6; ++k) {
+        float ms = 0.f;
+        err = hipEventElapsedTime(&ms, P->events[7 * x + k], P->events[7 * x + k + 1]);
+        if (err != hipSuccess) { rc = hip_fail(err, "hipEventElapsedTime"); break; }
+        phase_ms[k] += ms;
+      }
+  }
+  if (nexecs) *nexecs = n;
+  destroy_events(P);
+  return rc;
 }
 
 int bldfm_stability(const double* x, int64_t n, double* psi, double* phi, hipStream_t stream) {

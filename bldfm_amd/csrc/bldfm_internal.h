@@ -38,7 +38,10 @@ struct ExecBufs {
   double2* ws;         // complex workspace
 };
 
-int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream);
+// ev (optional): 7 events recorded at the phase boundaries
+// [level table | forward transform | shooting | Nyquist fix-up | y-pass | x-pass]
+int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream,
+                     hipEvent_t* ev = nullptr);
 int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream);
 int bldfm_launch_fft2(const FftPlan& F0, const double2* tw0, const FftPlan& F1, const double2* tw1,
                       int32_t batch, const double2* in, double2* mid, double2* out, int32_t sign,

@@ -496,11 +496,18 @@ static inline unsigned grid_for(int64_t tiles, int cap) {
     if (e_ != hipSuccess) return -(int)e_ - 1000; \
   } while (0)
 
-int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream) {
+int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream,
+                     hipEvent_t* ev) {
+#define MARK(k)                                            \
+  if (ev) {                                                \
+    if (hipEventRecord(ev[k], stream) != hipSuccess) return -1999; \
+  }
+  MARK(0)
   if (P.tot_tab > 0) {
     k_level_table<<<grid_for(P.tot_tab, grid_cap), 256, 0, stream>>>(P, B);
     LAUNCH_CHECK();
   }
+  MARK(1)
   const unsigned ft = (unsigned)P.fft_threads;
   const size_t lds = (size_t)P.fft_lds_bytes;
   if (P.any_dispersion) {
@@ -509,6 +516,7 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
     k_fwd_cols<<<grid_for(P.tot_fwd_cols, grid_cap), ft, lds, stream>>>(P, B);
     LAUNCH_CHECK();
   }
+  MARK(2)
   const unsigned gs = grid_for(P.tot_shoot, grid_cap);
 #define SHOOT(V, MAXS, RECOMP, EX)                                                    \
   if (P.variant_used[V][EX]) {                                                        \
@@ -526,14 +534,19 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
     k_shoot_analytic<<<gs, SHOOT_THREADS, 0, stream>>>(P, B, 3);
     LAUNCH_CHECK();
   }
+  MARK(3)
   if (P.tot_nyq > 0) {
     k_nyq_fix<<<grid_for(P.tot_nyq, grid_cap), 256, 0, stream>>>(P, B);
     LAUNCH_CHECK();
   }
+  MARK(4)
   k_inv_cols<<<grid_for(P.tot_inv_cols, grid_cap), ft, lds, stream>>>(P, B);
   LAUNCH_CHECK();
+  MARK(5)
   k_inv_rows<<<grid_for(P.tot_inv_rows, grid_cap), ft, lds, stream>>>(P, B);
   LAUNCH_CHECK();
+  MARK(6)
+#undef MARK
   return 0;
 }
 

@@ -160,6 +160,17 @@ int bldfm_plan_execute(bldfm_plan* plan, double* prof, const double* src, void* 
                        void* workspace, hipStream_t stream);
 
 /*
+ * Phase timing (no counterpart in the reference; for benchmarks): after
+ * bldfm_plan_timing_begin, the next max_execs executions record HIP events
+ * on their stream at the phase boundaries; bldfm_plan_timing_end waits for
+ * them and returns the summed milliseconds of the six phases
+ * [level table, forward transform, shooting, Nyquist fix-up, y-pass, x-pass]
+ * and the number of timed executions.  Not for use under graph capture.
+ */
+int bldfm_plan_timing_begin(bldfm_plan* plan, int32_t max_execs);
+int bldfm_plan_timing_end(bldfm_plan* plan, double* phase_ms, int32_t* nexecs);
+
+/*
  * 2-D complex FFT of `batch` contiguous (n0, n1) complex128 arrays with
  * numpy semantics: sign -1 (fft2) or +1 (ifft2), every output multiplied by
  * `scale`.  Replaces FFTManager.fft2/ifft2 (src/bldfm/fft_manager.py:78-102).
