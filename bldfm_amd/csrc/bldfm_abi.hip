@@ -95,7 +95,8 @@ void twiddles(int n, double2* w) {
 }
 
 int fft_threads_for(int n) {
-  int t = (n + kMaxE - 1) / kMaxE;
+  const int e = n <= BLDFM_FFT_SMALL_N ? 8 : kMaxE;
+  int t = (n + e - 1) / e;
   t = (int)align_up(std::max(t, 64), 64);
   return t;
 }
@@ -397,6 +398,7 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   V.any_dispersion = any_disp;
   for (int k = 0; k < 4; ++k)
     for (int e = 0; e < 2; ++e) V.variant_used[k][e] = variant_used[k][e];
+  V.fft_small = max_n <= BLDFM_FFT_SMALL_N ? 1 : 0;
   V.fft_threads = fft_threads_for(max_n);
   V.fft_lds_bytes = max_n * (int)sizeof(double2);
   V.tot_tab = tiles[0][ninst];

@@ -10,7 +10,9 @@
 //
 // W_n^t = e^{-2 pi i t/n} comes from a precomputed table; the e^{+i} transform
 // conjugates it.  A thread handles at most MAXE complex values per stage, so
-// the launch must use T >= ceil(n / MAXE) threads.
+// the launch must use T >= ceil(n / MAXE) threads.  Two configurations are
+// compiled: up to 256 threads with MAXE = 8 (n <= 2048, no register spills),
+// and up to 1024 threads with MAXE = BLDFM_FFT_MAXE for longer transforms.
 #pragma once
 #include "bldfm_math.h"
 
@@ -98,17 +100,23 @@ __device__ __forceinline__ void butterfly<8>(double2* x, bool inv) {
   x[7] = csub(e[3], o3);
 }
 
+// j / d for 0 <= j, d < 2^16 by multiply-high: with m = floor(2^32/d) + 1 the
+// error of j*m/2^32 against j/d is below 2^-16 < 1/d, so the floor is exact.
+__device__ __forceinline__ uint64_t div_magic(int d) { return (1ull << 32) / (uint64_t)d + 1ull; }
+__device__ __forceinline__ int fdiv(int j, uint64_t m) { return (int)(((uint64_t)(uint32_t)j * m) >> 32); }
+
 template <int R, int MAXB>
 __device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const double2* __restrict__ tw,
                                         bool inv, int tid, int T) {
   const int nb = n / R;
   const int stride = n / (Ns * R);
+  const uint64_t mNs = div_magic(Ns);
   double2 x[MAXB][R];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int j = tid + b * T;
     if (j < nb) {
-      const int k = j % Ns;
+      const int k = j - fdiv(j, mNs) * Ns;
 #pragma unroll
       for (int r = 0; r < R; ++r) x[b][r] = buf[j + r * nb];
       if (k != 0) {
@@ -127,8 +135,9 @@ __device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const doubl
   for (int b = 0; b < MAXB; ++b) {
     const int j = tid + b * T;
     if (j < nb) {
-      const int k = j % Ns;
-      const int base = (j - k) * R + k;
+      const int q = fdiv(j, mNs);
+      const int k = j - q * Ns;
+      const int base = q * Ns * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[base + r * Ns] = x[b][r];
     }
@@ -137,19 +146,23 @@ __device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const doubl
 }
 
 // Any radix: one output per (thread, slot), out[o] = sum_m x_m W_n^{m*stride*(k + r*Ns)}.
+template <int MAXE>
 __device__ __forceinline__ void stage_generic(double2* buf, int n, int R, int Ns,
                                               const double2* __restrict__ tw, bool inv, int tid,
                                               int T) {
   const int nb = n / R;
   const int stride = n / (Ns * R);
-  double2 y[BLDFM_FFT_MAXE];
+  const uint64_t mNs = div_magic(Ns), mR = div_magic(R);
+  double2 y[MAXE];
 #pragma unroll
-  for (int e = 0; e < BLDFM_FFT_MAXE; ++e) {
+  for (int e = 0; e < MAXE; ++e) {
     const int o = tid + e * T;
     if (o < n) {
-      const int k = o % Ns;
-      const int r = (o / Ns) % R;
-      const int j = (o / (Ns * R)) * Ns + k;
+      const int q = fdiv(o, mNs);      // o / Ns
+      const int k = o - q * Ns;        // o mod Ns
+      const int g = fdiv(q, mR);       // o / (Ns R)
+      const int r = q - g * R;         // (o / Ns) mod R
+      const int j = g * Ns + k;
       const int step = stride * (k + r * Ns);
       double2 acc = cmk(0.0, 0.0);
       int t = 0;
@@ -165,14 +178,16 @@ __device__ __forceinline__ void stage_generic(double2* buf, int n, int R, int Ns
   }
   __syncthreads();
 #pragma unroll
-  for (int e = 0; e < BLDFM_FFT_MAXE; ++e) {
+  for (int e = 0; e < MAXE; ++e) {
     const int o = tid + e * T;
     if (o < n) buf[o] = y[e];
   }
   __syncthreads();
 }
 
-// Full transform on buf (n points) with plan P; inv selects e^{+i}.
+// Full transform on buf (n points) with plan P; inv selects e^{+i}.  A thread
+// handles at most MAXE values per stage: the launch uses T >= ceil(n/MAXE).
+template <int MAXE>
 __device__ inline void lds_fft(double2* buf, const FftPlan& P, const double2* __restrict__ tw,
                                bool inv, int tid, int T) {
   int Ns = 1;
@@ -180,12 +195,12 @@ __device__ inline void lds_fft(double2* buf, const FftPlan& P, const double2* __
   for (int s = 0; s < P.nstage; ++s) {
     const int R = P.radix[s];
     switch (R) {
-      case 2: stage_r<2, BLDFM_FFT_MAXE / 2>(buf, n, Ns, tw, inv, tid, T); break;
-      case 3: stage_r<3, (BLDFM_FFT_MAXE + 2) / 3>(buf, n, Ns, tw, inv, tid, T); break;
-      case 4: stage_r<4, BLDFM_FFT_MAXE / 4>(buf, n, Ns, tw, inv, tid, T); break;
-      case 5: stage_r<5, (BLDFM_FFT_MAXE + 4) / 5>(buf, n, Ns, tw, inv, tid, T); break;
-      case 8: stage_r<8, BLDFM_FFT_MAXE / 8>(buf, n, Ns, tw, inv, tid, T); break;
-      default: stage_generic(buf, n, R, Ns, tw, inv, tid, T); break;
+      case 2: stage_r<2, (MAXE + 1) / 2>(buf, n, Ns, tw, inv, tid, T); break;
+      case 3: stage_r<3, (MAXE + 2) / 3>(buf, n, Ns, tw, inv, tid, T); break;
+      case 4: stage_r<4, (MAXE + 3) / 4>(buf, n, Ns, tw, inv, tid, T); break;
+      case 5: stage_r<5, (MAXE + 4) / 5>(buf, n, Ns, tw, inv, tid, T); break;
+      case 8: stage_r<8, (MAXE + 7) / 8>(buf, n, Ns, tw, inv, tid, T); break;
+      default: stage_generic<MAXE>(buf, n, R, Ns, tw, inv, tid, T); break;
     }
     Ns *= R;
   }

@@ -24,6 +24,7 @@ struct PlanDev {
   int32_t ninst;
   int32_t any_dispersion;
   int32_t variant_used[4][2]; // shooting kernels this plan launches [variant][exact]
+  int32_t fft_small;       // all transforms <= BLDFM_FFT_SMALL_N: 256-thread kernels
   int32_t fft_threads;     // threads per FFT workgroup
   int32_t fft_lds_bytes;   // dynamic LDS per FFT workgroup
   int64_t tot_tab, tot_fwd_rows, tot_fwd_cols, tot_shoot, tot_nyq, tot_inv_cols, tot_inv_rows;

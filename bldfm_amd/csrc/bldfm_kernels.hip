@@ -18,6 +18,7 @@
 #include "bldfm_fft.h"
 
 #define SHOOT_THREADS 256
+#define FFT_SMALL_E 8     // values per thread per stage, 256-thread FFT kernels (n <= 2048)
 
 __device__ __forceinline__ int find_inst(const int64_t* __restrict__ off, int ninst, int64_t t) {
   int lo = 0, hi = ninst - 1;
@@ -163,26 +164,42 @@ __device__ __forceinline__ double2 ivp2_init(const InstDesc& D, const Lane& L,
 //   T = (-(Kx*lx2 + Ky*ly2), -(u*lx + v*ly))
 //   a = 1 - (sa*T)*h2,  b = b0 - (sb*T)*h3,  c = T*h - (sc*T^2)*h3
 // otherwise: the same algebra with fused multiply-adds.
+// The level table and slot map are uniform across a workgroup and read-only
+// during the march: reading them through the constant address space lets
+// the compiler fetch them with scalar loads into SGPRs.
+typedef const __attribute__((address_space(4))) double ctab_t;
+typedef const __attribute__((address_space(4))) int32_t cint_t;
+
+// One level-table row in registers (SGPRs: the row is uniform).
+struct Row {
+  double kx, ky, u, v, sa, h2, b0, sb, h3, h, sc, A, B1, C;
+};
+
+__device__ __forceinline__ Row load_row(ctab_t* __restrict__ T) {
+  Row r;
+  r.kx = T[0]; r.ky = T[1]; r.u = T[2]; r.v = T[3];
+  r.sa = T[4]; r.h2 = T[5]; r.b0 = T[6]; r.sb = T[7]; r.h3 = T[8]; r.h = T[9]; r.sc = T[10];
+  r.A = T[12]; r.B1 = T[13]; r.C = T[14];
+  return r;
+}
+
 template <bool EXACT>
-__device__ __forceinline__ void march_step(const double* __restrict__ T, double lx, double ly,
-                                           double lx2, double ly2, double2& p1, double2& q1,
-                                           double2& p2, double2& q2);
+__device__ __forceinline__ void march_step(const Row& R, double lx, double ly, double lx2, double ly2,
+                                           double2& p1, double2& q1, double2& p2, double2& q2);
 
 template <>
-__device__ __forceinline__ void march_step<true>(const double* __restrict__ T, double lx, double ly,
-                                                 double lx2, double ly2, double2& p1, double2& q1,
-                                                 double2& p2, double2& q2) {
+__device__ __forceinline__ void march_step<true>(const Row& R, double lx, double ly, double lx2,
+                                                 double ly2, double2& p1, double2& q1, double2& p2,
+                                                 double2& q2) {
 #pragma clang fp contract(off)
-  const double kx = T[0], ky = T[1], u = T[2], v = T[3];
-  const double sa = T[4], h2 = T[5], b0 = T[6], sb = T[7], h3 = T[8], h = T[9], sc = T[10];
-  const double tr = -(kx * lx2 + ky * ly2);
-  const double ti = -(u * lx + v * ly);
-  const double ar = 1.0 - (sa * tr) * h2, ai = -((sa * ti) * h2);
-  const double br = b0 - (sb * tr) * h3, bi = -((sb * ti) * h3);
+  const double tr = -(R.kx * lx2 + R.ky * ly2);
+  const double ti = -(R.u * lx + R.v * ly);
+  const double ar = 1.0 - (R.sa * tr) * R.h2, ai = -((R.sa * ti) * R.h2);
+  const double br = R.b0 - (R.sb * tr) * R.h3, bi = -((R.sb * ti) * R.h3);
   const double t2r = tr * tr - ti * ti;
   const double tri = tr * ti;
   const double t2i = tri + tri;
-  const double cr = tr * h - (sc * t2r) * h3, ci = ti * h - (sc * t2i) * h3;
+  const double cr = tr * R.h - (R.sc * t2r) * R.h3, ci = ti * R.h - (R.sc * t2i) * R.h3;
   const double2 a = cmk(ar, ai), b = cmk(br, bi), c = cmk(cr, ci);
   const double2 np1 = xcmuladd(a, p1, b, q1);
   const double2 nq1 = xcmuladd(c, p1, a, q1);
@@ -192,16 +209,15 @@ __device__ __forceinline__ void march_step<true>(const double* __restrict__ T, d
 }
 
 template <>
-__device__ __forceinline__ void march_step<false>(const double* __restrict__ T, double lx, double ly,
-                                                  double lx2, double ly2, double2& p1, double2& q1,
-                                                  double2& p2, double2& q2) {
-  const double tr = -(T[0] * lx2 + T[1] * ly2);
-  const double ti = -(T[2] * lx + T[3] * ly);
-  const double A = T[12], B0 = T[6], B1 = T[13], H = T[9], C = T[14];
-  const double2 a = cmk(1.0 - A * tr, -(A * ti));
-  const double2 b = cmk(B0 - B1 * tr, -(B1 * ti));
+__device__ __forceinline__ void march_step<false>(const Row& R, double lx, double ly, double lx2,
+                                                  double ly2, double2& p1, double2& q1, double2& p2,
+                                                  double2& q2) {
+  const double tr = -(R.kx * lx2 + R.ky * ly2);
+  const double ti = -(R.u * lx + R.v * ly);
+  const double2 a = cmk(1.0 - R.A * tr, -(R.A * ti));
+  const double2 b = cmk(R.b0 - R.B1 * tr, -(R.B1 * ti));
   const double t2r = tr * tr - ti * ti, t2i = 2.0 * tr * ti;
-  const double2 c = cmk(H * tr - C * t2r, H * ti - C * t2i);
+  const double2 c = cmk(R.h * tr - R.C * t2r, R.h * ti - R.C * t2i);
   const double2 np1 = cadd(cmul(a, p1), cmul(b, q1));
   const double2 nq1 = cadd(cmul(c, p1), cmul(a, q1));
   const double2 np2 = cadd(cmul(a, p2), cmul(b, q2));
@@ -257,10 +273,11 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
     const int lane = (int)(t - P.t_shoot[inst]) * SHOOT_THREADS + threadIdx.x;
     if (!lane_setup(D, lane, L)) continue;
     const double* __restrict__ tab = B.tab + D.off_tab;
-    const int32_t* __restrict__ slot = P.ints + D.off_slot;
+    ctab_t* __restrict__ ctab = (ctab_t*)tab;
+    cint_t* __restrict__ slot = (cint_t*)(P.ints + D.off_slot);
     const int nz = D.nz;
     if (L.kind == LANE_ORIGIN) {
-      origin_numeric(D, tab, slot, B.ws);
+      origin_numeric(D, tab, P.ints + D.off_slot, B.ws);
       continue;
     }
     const double lx = L.lx, ly = L.ly, lx2 = xmul(lx, lx), ly2 = xmul(ly, ly);
@@ -269,6 +286,8 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
     double2 sp1[MAXS], sq1[MAXS], sp2[MAXS], sq2[MAXS];
 #pragma unroll
     for (int s = 0; s < MAXS; ++s) sp1[s] = sq1[s] = sp2[s] = sq2[s] = cmk(0.0, 0.0);
+    // the next row is fetched while the current step computes
+    Row nxt = load_row(ctab);
     for (int i = 0; i < nz - 1; ++i) {
       if (!RECOMP) {
         const int sl = slot[i];
@@ -278,7 +297,9 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
             if (s == sl) { sp1[s] = p1; sq1[s] = q1; sp2[s] = p2; sq2[s] = q2; }
         }
       }
-      march_step<EXACT>(tab + (int64_t)i * BLDFM_LEVEL_STRIDE, lx, ly, lx2, ly2, p1, q1, p2, q2);
+      const Row cur = nxt;
+      if (i + 1 < nz - 1) nxt = load_row(ctab + (int64_t)(i + 1) * BLDFM_LEVEL_STRIDE);
+      march_step<EXACT>(cur, lx, ly, lx2, ly2, p1, q1, p2, q2);
     }
     if (!RECOMP) {
       const int sl = slot[nz - 1];
@@ -299,12 +320,16 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
                     xcadd(ncmul(alpha, sq1[s]), sq2[s]), sh);
     } else {
       p1 = cmk(1.0, 0.0); q1 = cmk(0.0, 0.0); p2 = cmk(0.0, 0.0); q2 = q0;
+      Row nxt2 = load_row(ctab);
       for (int i = 0; i <= D.max_level; ++i) {
         const int sl = slot[i];
         if (sl >= 0)
           emit_mode(D, L, B.ws, sl, xcadd(ncmul(alpha, p1), p2), xcadd(ncmul(alpha, q1), q2), sh);
-        if (i < D.max_level)
-          march_step<EXACT>(tab + (int64_t)i * BLDFM_LEVEL_STRIDE, lx, ly, lx2, ly2, p1, q1, p2, q2);
+        if (i < D.max_level) {
+          const Row cur = nxt2;
+          if (i + 1 < D.max_level) nxt2 = load_row(ctab + (int64_t)(i + 1) * BLDFM_LEVEL_STRIDE);
+          march_step<EXACT>(cur, lx, ly, lx2, ly2, p1, q1, p2, q2);
+        }
       }
     }
     for (int s = D.nfilled; s < D.nslot; ++s) emit_mode(D, L, B.ws, s, cmk(0.0, 0.0), cmk(0.0, 0.0), sh);
@@ -385,7 +410,8 @@ __device__ __forceinline__ void lds_zero(int n, int tid, int T) {
 // Forward, rows: one surface-flux row y placed at [px, px+nx) of nxe points,
 // e^{-i} DFT, scale 1/nxe (numpy norm="forward"), keep the nlx retained
 // columns in truncated order.
-__global__ void k_fwd_rows(PlanDev P, ExecBufs B) {
+template <int MAXE, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t t = blockIdx.x; t < P.tot_fwd_rows; t += gridDim.x) {
     const int inst = find_inst(P.t_fwd_rows, P.ninst, t);
@@ -396,7 +422,7 @@ __global__ void k_fwd_rows(PlanDev P, ExecBufs B) {
     const double* row = B.src + D.off_src + (int64_t)y * D.nx;
     for (int x = tid; x < D.nx; x += T) smem[D.px + x] = cmk(row[x], 0.0);
     __syncthreads();
-    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, false, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, false, tid, T);
     const double fct = 1.0 / (double)F.n;
     double2* aw = B.ws + D.off_aw + (int64_t)y * D.nlx;
     for (int j = tid; j < D.nlx; j += T)
@@ -407,7 +433,8 @@ __global__ void k_fwd_rows(PlanDev P, ExecBufs B) {
 
 // Forward, columns: retained column jx over y placed at [py, py+ny) of nye
 // points, e^{-i} DFT, scale 1/nye, keep the nly retained rows -> q^0 (x-major).
-__global__ void k_fwd_cols(PlanDev P, ExecBufs B) {
+template <int MAXE, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t t = blockIdx.x; t < P.tot_fwd_cols; t += gridDim.x) {
     const int inst = find_inst(P.t_fwd_cols, P.ninst, t);
@@ -418,7 +445,7 @@ __global__ void k_fwd_cols(PlanDev P, ExecBufs B) {
     const double2* aw = B.ws + D.off_aw;
     for (int y = tid; y < D.ny; y += T) smem[D.py + y] = aw[(int64_t)y * D.nlx + jx];
     __syncthreads();
-    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, false, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, false, tid, T);
     const double fct = 1.0 / (double)F.n;
     double2* q0 = B.ws + D.off_q0 + (int64_t)jx * D.nly;
     for (int j = tid; j < D.nly; j += T)
@@ -430,7 +457,8 @@ __global__ void k_fwd_cols(PlanDev P, ExecBufs B) {
 // Inverse, y-pass: band column cx of slot s (compact rows scattered to their
 // padded indices), unnormalised DFT (e^{-i} footprint, e^{+i} dispersion),
 // keep the cropped rows [py, py+ny).
-__global__ void k_inv_cols(PlanDev P, ExecBufs B) {
+template <int MAXE, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t t = blockIdx.x; t < P.tot_inv_cols; t += gridDim.x) {
     const int inst = find_inst(P.t_inv_cols, P.ninst, t);
@@ -441,7 +469,7 @@ __global__ void k_inv_cols(PlanDev P, ExecBufs B) {
     const double2* zc = B.ws + D.off_z + local * D.nyb;
     for (int c = tid; c < D.nyb; c += T) smem[padded_of_compact(c, D.nyb, F.n)] = zc[c];
     __syncthreads();
-    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
     double2* v = B.ws + D.off_v + local * D.ny;
     for (int y = tid; y < D.ny; y += T) v[y] = smem[D.py + y];
     __syncthreads();
@@ -450,7 +478,8 @@ __global__ void k_inv_cols(PlanDev P, ExecBufs B) {
 
 // Inverse, x-pass: cropped row y of slot s over the band columns, DFT,
 // keep [px, px+nx); conc = Re, flx = Im.
-__global__ void k_inv_rows(PlanDev P, ExecBufs B) {
+template <int MAXE, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t t = blockIdx.x; t < P.tot_inv_rows; t += gridDim.x) {
     const int inst = find_inst(P.t_inv_rows, P.ninst, t);
@@ -462,7 +491,7 @@ __global__ void k_inv_rows(PlanDev P, ExecBufs B) {
     const double2* v = B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y;
     for (int c = tid; c < D.nxb; c += T) smem[padded_of_compact(c, D.nxb, F.n)] = v[(int64_t)c * D.ny];
     __syncthreads();
-    bldfm_fft::lds_fft(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
     const int64_t o = D.off_out + local * D.nx;
     if (D.out_f32) {
       float* conc = (float*)B.conc + o;
@@ -510,11 +539,13 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
   MARK(1)
   const unsigned ft = (unsigned)P.fft_threads;
   const size_t lds = (size_t)P.fft_lds_bytes;
+#define FFT_LAUNCH(K, tiles)                                                     \
+  if (P.fft_small) K<FFT_SMALL_E, 256><<<grid_for(tiles, grid_cap), ft, lds, stream>>>(P, B); \
+  else K<BLDFM_FFT_MAXE, 1024><<<grid_for(tiles, grid_cap), ft, lds, stream>>>(P, B);        \
+  LAUNCH_CHECK();
   if (P.any_dispersion) {
-    k_fwd_rows<<<grid_for(P.tot_fwd_rows, grid_cap), ft, lds, stream>>>(P, B);
-    LAUNCH_CHECK();
-    k_fwd_cols<<<grid_for(P.tot_fwd_cols, grid_cap), ft, lds, stream>>>(P, B);
-    LAUNCH_CHECK();
+    FFT_LAUNCH(k_fwd_rows, P.tot_fwd_rows)
+    FFT_LAUNCH(k_fwd_cols, P.tot_fwd_cols)
   }
   MARK(2)
   const unsigned gs = grid_for(P.tot_shoot, grid_cap);
@@ -540,12 +571,11 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
     LAUNCH_CHECK();
   }
   MARK(4)
-  k_inv_cols<<<grid_for(P.tot_inv_cols, grid_cap), ft, lds, stream>>>(P, B);
-  LAUNCH_CHECK();
+  FFT_LAUNCH(k_inv_cols, P.tot_inv_cols)
   MARK(5)
-  k_inv_rows<<<grid_for(P.tot_inv_rows, grid_cap), ft, lds, stream>>>(P, B);
-  LAUNCH_CHECK();
+  FFT_LAUNCH(k_inv_rows, P.tot_inv_rows)
   MARK(6)
+#undef FFT_LAUNCH
 #undef MARK
   return 0;
 }
@@ -571,21 +601,23 @@ int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream) {
 // numpy fft2/ifft2 on `batch` contiguous (n0, n1) complex arrays: x-pass
 // over the rows into the workspace, then y-pass over the columns, output
 // multiplied by `scale`.
-__global__ void k_fft2_rows(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ in,
+template <int MAXE, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_fft2_rows(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ in,
                             double2* __restrict__ mid, int64_t nrows, bool inv) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
     const double2* src = in + r * F.n;
     for (int i = tid; i < F.n; i += T) smem[i] = src[i];
     __syncthreads();
-    bldfm_fft::lds_fft(smem, F, tw, inv, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, F, tw, inv, tid, T);
     double2* dst = mid + r * F.n;
     for (int i = tid; i < F.n; i += T) dst[i] = smem[i];
     __syncthreads();
   }
 }
 
-__global__ void k_fft2_cols(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ mid,
+template <int MAXE, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_fft2_cols(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ mid,
                             double2* __restrict__ out, int32_t n1, int64_t ncols, double scale,
                             bool inv) {
   const int T = blockDim.x, tid = threadIdx.x;
@@ -595,7 +627,7 @@ __global__ void k_fft2_cols(FftPlan F, const double2* __restrict__ tw, const dou
     const double2* src = mid + b * (int64_t)n0 * n1 + col;
     for (int i = tid; i < n0; i += T) smem[i] = src[(int64_t)i * n1];
     __syncthreads();
-    bldfm_fft::lds_fft(smem, F, tw, inv, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, F, tw, inv, tid, T);
     double2* dst = out + b * (int64_t)n0 * n1 + col;
     for (int i = tid; i < n0; i += T) dst[(int64_t)i * n1] = cscale(scale, smem[i]);
     __syncthreads();
@@ -607,18 +639,29 @@ int bldfm_launch_fft2(const FftPlan& F0, const double2* tw0, const FftPlan& F1, 
                       double scale, int threads, int lds_bytes, int grid_cap, hipStream_t stream) {
   const bool inv = sign > 0;
   const int64_t nrows = (int64_t)batch * F0.n, ncols = (int64_t)batch * F1.n;
-  k_fft2_rows<<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, nrows, inv);
-  LAUNCH_CHECK();
-  k_fft2_cols<<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, F1.n, ncols,
-                                                                          scale, inv);
-  LAUNCH_CHECK();
+  if (threads <= 256) {
+    k_fft2_rows<FFT_SMALL_E, 256><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, nrows, inv);
+    LAUNCH_CHECK();
+    k_fft2_cols<FFT_SMALL_E, 256><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, F1.n, ncols, scale, inv);
+    LAUNCH_CHECK();
+  } else {
+    k_fft2_rows<BLDFM_FFT_MAXE, 1024><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, nrows, inv);
+    LAUNCH_CHECK();
+    k_fft2_cols<BLDFM_FFT_MAXE, 1024><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, F1.n, ncols, scale, inv);
+    LAUNCH_CHECK();
+  }
   return 0;
 }
 
 // Allow dynamic LDS beyond the default per-workgroup cap for the FFT kernels.
 void bldfm_set_fft_lds_attr(int bytes) {
-  const void* fns[] = {(const void*)k_fwd_rows, (const void*)k_fwd_cols, (const void*)k_inv_cols,
-                       (const void*)k_inv_rows, (const void*)k_fft2_rows, (const void*)k_fft2_cols};
+  const void* fns[] = {
+      (const void*)k_fwd_rows<FFT_SMALL_E, 256>, (const void*)k_fwd_cols<FFT_SMALL_E, 256>,
+      (const void*)k_inv_cols<FFT_SMALL_E, 256>, (const void*)k_inv_rows<FFT_SMALL_E, 256>,
+      (const void*)k_fft2_rows<FFT_SMALL_E, 256>, (const void*)k_fft2_cols<FFT_SMALL_E, 256>,
+      (const void*)k_fwd_rows<BLDFM_FFT_MAXE, 1024>, (const void*)k_fwd_cols<BLDFM_FFT_MAXE, 1024>,
+      (const void*)k_inv_cols<BLDFM_FFT_MAXE, 1024>, (const void*)k_inv_rows<BLDFM_FFT_MAXE, 1024>,
+      (const void*)k_fft2_rows<BLDFM_FFT_MAXE, 1024>, (const void*)k_fft2_cols<BLDFM_FFT_MAXE, 1024>};
   for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -26,7 +26,8 @@
 
 #define BLDFM_MAX_RADIX_STAGES 24
 #define BLDFM_LEVEL_STRIDE 16  // doubles per vertical step in the level table
-#define BLDFM_FFT_MAXE 16      // max complex values per thread per FFT stage
+#define BLDFM_FFT_MAXE 10      // values per thread per FFT stage, 1024-thread kernels (n <= 10240)
+#define BLDFM_FFT_SMALL_N 2048 // longest transform of the 256-thread kernels (8 values per thread)
 
 // Mixed-radix Stockham plan for one transform length.
 struct FftPlan {
