@@ -118,24 +118,33 @@ int check_fft_len(int n, int device, const char* what) {
   return 0;
 }
 
-// Twiddle tables shared by plain 2-D FFT calls, per (device, n).
+// Plans of the plain 2-D FFT, per (device, n): one device block holding the
+// FftPlan followed by its n twiddles.
 std::mutex g_tw_mu;
-std::map<std::pair<int, int>, double2*> g_tw_cache;
+std::map<std::pair<int, int>, char*> g_tw_cache;
+constexpr size_t kPlanHdr = 256;  // FftPlan padded to 256 bytes
+static_assert(sizeof(FftPlan) <= kPlanHdr, "FftPlan must fit its header");
 
-int cached_twiddles(int device, int n, const double2** out) {
+int cached_plan(int device, int n, const FftPlan** plan, const double2** tw) {
   std::lock_guard<std::mutex> lk(g_tw_mu);
   auto key = std::make_pair(device, n);
   auto it = g_tw_cache.find(key);
-  if (it != g_tw_cache.end()) { *out = it->second; return 0; }
-  std::vector<double2> h(n);
-  twiddles(n, h.data());
-  double2* d = nullptr;
-  hipError_t e = hipMalloc(&d, sizeof(double2) * n);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
-  e = hipMemcpy(d, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
-  if (e != hipSuccess) { (void)hipFree(d); return hip_fail(e, "hipMemcpy(twiddles)"); }
-  g_tw_cache[key] = d;
-  *out = d;
+  if (it == g_tw_cache.end()) {
+    FftPlan F;
+    if (!factorize(n, &F)) return fail(BLDFM_ELIMIT, "too many FFT stages");
+    F.tw_off = 0;
+    std::vector<char> h(kPlanHdr + sizeof(double2) * n, 0);
+    memcpy(h.data(), &F, sizeof(F));
+    twiddles(n, (double2*)(h.data() + kPlanHdr));
+    char* d = nullptr;
+    hipError_t e = hipMalloc((void**)&d, h.size());
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(fft plan)");
+    e = hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(d); return hip_fail(e, "hipMemcpy(fft plan)"); }
+    it = g_tw_cache.emplace(key, d).first;
+  }
+  *plan = (const FftPlan*)it->second;
+  *tw = (const double2*)(it->second + kPlanHdr);
   return 0;
 }
 
@@ -551,16 +560,14 @@ int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* 
   int rc = check_fft_len(n0, device, "fft2 axis 0 length");
   if (!rc) rc = check_fft_len(n1, device, "fft2 axis 1 length");
   if (rc) return rc;
-  FftPlan F0, F1;
-  if (!factorize(n0, &F0) || !factorize(n1, &F1)) return fail(BLDFM_ELIMIT, "too many FFT stages");
-  F0.tw_off = F1.tw_off = 0;
+  const FftPlan *F0 = nullptr, *F1 = nullptr;
   const double2 *tw0 = nullptr, *tw1 = nullptr;
-  rc = cached_twiddles(device, n0, &tw0);
-  if (!rc) rc = cached_twiddles(device, n1, &tw1);
+  rc = cached_plan(device, n0, &F0, &tw0);
+  if (!rc) rc = cached_plan(device, n1, &F1, &tw1);
   if (rc) return rc;
   const int nmax = std::max(n0, n1);
   set_lds_attr(nmax * 16);
-  rc = bldfm_launch_fft2(F0, tw0, F1, tw1, batch, (const double2*)in, (double2*)workspace,
+  rc = bldfm_launch_fft2(F0, n0, tw0, F1, n1, tw1, batch, (const double2*)in, (double2*)workspace,
                          (double2*)out, sign, scale, fft_threads_for(nmax), nmax * 16, kGridCap, stream);
   if (rc) return fail(BLDFM_EHIP, std::string("fft2 launch failed: ") +
                                       hipGetErrorString((hipError_t)(-(rc + 1000))));

@@ -185,15 +185,20 @@ __device__ __forceinline__ void stage_generic(double2* buf, int n, int R, int Ns
   __syncthreads();
 }
 
-// Full transform on buf (n points) with plan P; inv selects e^{+i}.  A thread
-// handles at most MAXE values per stage: the launch uses T >= ceil(n/MAXE).
+// Full transform on buf (n points) with the plan at Pg (device memory,
+// read through the constant address space: the plan is uniform);
+// inv selects e^{+i}.  A thread handles at most MAXE values per stage: the
+// launch uses T >= ceil(n/MAXE).
 template <int MAXE>
-__device__ inline void lds_fft(double2* buf, const FftPlan& P, const double2* __restrict__ tw,
-                               bool inv, int tid, int T) {
+__device__ __forceinline__ void lds_fft(double2* buf, const FftPlan* Pg, const double2* __restrict__ tw,
+                                        bool inv, int tid, int T) {
+  typedef const __attribute__((address_space(4))) int32_t cint32;
+  const cint32* pc = (const cint32*)Pg;
+  const int n = pc[0];
+  const int nstage = pc[1];
   int Ns = 1;
-  const int n = P.n;
-  for (int s = 0; s < P.nstage; ++s) {
-    const int R = P.radix[s];
+  for (int s = 0; s < nstage; ++s) {
+    const int R = pc[2 + s];
     switch (R) {
       case 2: stage_r<2, (MAXE + 1) / 2>(buf, n, Ns, tw, inv, tid, T); break;
       case 3: stage_r<3, (MAXE + 2) / 3>(buf, n, Ns, tw, inv, tid, T); break;

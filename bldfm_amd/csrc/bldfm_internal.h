@@ -44,9 +44,11 @@ struct ExecBufs {
 int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream,
                      hipEvent_t* ev = nullptr);
 int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream);
-int bldfm_launch_fft2(const FftPlan& F0, const double2* tw0, const FftPlan& F1, const double2* tw1,
-                      int32_t batch, const double2* in, double2* mid, double2* out, int32_t sign,
-                      double scale, int threads, int lds_bytes, int grid_cap, hipStream_t stream);
+// F0/F1: device-resident plans of the two axis lengths n0 (columns) and n1 (rows)
+int bldfm_launch_fft2(const FftPlan* F0, int32_t n0, const double2* tw0, const FftPlan* F1, int32_t n1,
+                      const double2* tw1, int32_t batch, const double2* in, double2* mid, double2* out,
+                      int32_t sign, double scale, int threads, int lds_bytes, int grid_cap,
+                      hipStream_t stream);
 void bldfm_set_fft_lds_attr(int bytes);
 int bldfm_launch_stability(const double* x, int64_t n, double* psi, double* phi, hipStream_t stream);
 int bldfm_launch_wind(const double* spd, const double* dir, int64_t n, double* u, double* v, hipStream_t stream);

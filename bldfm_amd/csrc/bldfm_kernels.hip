@@ -20,6 +20,14 @@
 #define SHOOT_THREADS 256
 #define FFT_SMALL_E 8     // values per thread per stage, 256-thread FFT kernels (n <= 2048)
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  This bijection of [0, n) gives every XCD a contiguous range of tiles,
+// so neighbouring tiles that share cache lines run behind the same L2.
+__device__ __forceinline__ int64_t xcd_swizzle(int64_t b, int64_t n) {
+  const int64_t q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+  return x * q + (x < r ? x : r) + i;
+}
+
 __device__ __forceinline__ int find_inst(const int64_t* __restrict__ off, int ninst, int64_t t) {
   int lo = 0, hi = ninst - 1;
   while (lo < hi) {
@@ -413,7 +421,8 @@ __device__ __forceinline__ void lds_zero(int n, int tid, int T) {
 template <int MAXE, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t t = blockIdx.x; t < P.tot_fwd_rows; t += gridDim.x) {
+  for (int64_t b = blockIdx.x; b < P.tot_fwd_rows; b += gridDim.x) {
+    const int64_t t = gridDim.x >= P.tot_fwd_rows ? xcd_swizzle(b, P.tot_fwd_rows) : b;
     const int inst = find_inst(P.t_fwd_rows, P.ninst, t);
     const InstDesc& D = P.desc[inst];
     const int y = (int)(t - P.t_fwd_rows[inst]);
@@ -422,7 +431,7 @@ __global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
     const double* row = B.src + D.off_src + (int64_t)y * D.nx;
     for (int x = tid; x < D.nx; x += T) smem[D.px + x] = cmk(row[x], 0.0);
     __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, false, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, false, tid, T);
     const double fct = 1.0 / (double)F.n;
     double2* aw = B.ws + D.off_aw + (int64_t)y * D.nlx;
     for (int j = tid; j < D.nlx; j += T)
@@ -436,7 +445,8 @@ __global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
 template <int MAXE, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t t = blockIdx.x; t < P.tot_fwd_cols; t += gridDim.x) {
+  for (int64_t b = blockIdx.x; b < P.tot_fwd_cols; b += gridDim.x) {
+    const int64_t t = gridDim.x >= P.tot_fwd_cols ? xcd_swizzle(b, P.tot_fwd_cols) : b;
     const int inst = find_inst(P.t_fwd_cols, P.ninst, t);
     const InstDesc& D = P.desc[inst];
     const int jx = (int)(t - P.t_fwd_cols[inst]);
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
     const double2* aw = B.ws + D.off_aw;
     for (int y = tid; y < D.ny; y += T) smem[D.py + y] = aw[(int64_t)y * D.nlx + jx];
     __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, false, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, false, tid, T);
     const double fct = 1.0 / (double)F.n;
     double2* q0 = B.ws + D.off_q0 + (int64_t)jx * D.nly;
     for (int j = tid; j < D.nly; j += T)
@@ -460,7 +470,8 @@ __global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
 template <int MAXE, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t t = blockIdx.x; t < P.tot_inv_cols; t += gridDim.x) {
+  for (int64_t b = blockIdx.x; b < P.tot_inv_cols; b += gridDim.x) {
+    const int64_t t = gridDim.x >= P.tot_inv_cols ? xcd_swizzle(b, P.tot_inv_cols) : b;
     const int inst = find_inst(P.t_inv_cols, P.ninst, t);
     const InstDesc& D = P.desc[inst];
     const int64_t local = t - P.t_inv_cols[inst];  // s*nxb + cx
@@ -469,7 +480,7 @@ __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
     const double2* zc = B.ws + D.off_z + local * D.nyb;
     for (int c = tid; c < D.nyb; c += T) smem[padded_of_compact(c, D.nyb, F.n)] = zc[c];
     __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, !D.footprint, tid, T);
     double2* v = B.ws + D.off_v + local * D.ny;
     for (int y = tid; y < D.ny; y += T) v[y] = smem[D.py + y];
     __syncthreads();
@@ -481,7 +492,8 @@ __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
 template <int MAXE, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t t = blockIdx.x; t < P.tot_inv_rows; t += gridDim.x) {
+  for (int64_t b = blockIdx.x; b < P.tot_inv_rows; b += gridDim.x) {
+    const int64_t t = gridDim.x >= P.tot_inv_rows ? xcd_swizzle(b, P.tot_inv_rows) : b;
     const int inst = find_inst(P.t_inv_rows, P.ninst, t);
     const InstDesc& D = P.desc[inst];
     const int64_t local = t - P.t_inv_rows[inst];  // s*ny + y
@@ -491,7 +503,7 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
     const double2* v = B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y;
     for (int c = tid; c < D.nxb; c += T) smem[padded_of_compact(c, D.nxb, F.n)] = v[(int64_t)c * D.ny];
     __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, F, P.tw + F.tw_off, !D.footprint, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, !D.footprint, tid, T);
     const int64_t o = D.off_out + local * D.nx;
     if (D.out_f32) {
       float* conc = (float*)B.conc + o;
@@ -602,26 +614,26 @@ int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream) {
 // over the rows into the workspace, then y-pass over the columns, output
 // multiplied by `scale`.
 template <int MAXE, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_fft2_rows(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ in,
-                            double2* __restrict__ mid, int64_t nrows, bool inv) {
+__global__ __launch_bounds__(MAXT) void k_fft2_rows(const FftPlan* F, const double2* __restrict__ tw,
+                                                  const double2* __restrict__ in, double2* __restrict__ mid,
+                                                  int32_t n, int64_t nrows, bool inv) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
-    const double2* src = in + r * F.n;
-    for (int i = tid; i < F.n; i += T) smem[i] = src[i];
+    const double2* src = in + r * n;
+    for (int i = tid; i < n; i += T) smem[i] = src[i];
     __syncthreads();
     bldfm_fft::lds_fft<MAXE>(smem, F, tw, inv, tid, T);
-    double2* dst = mid + r * F.n;
-    for (int i = tid; i < F.n; i += T) dst[i] = smem[i];
+    double2* dst = mid + r * n;
+    for (int i = tid; i < n; i += T) dst[i] = smem[i];
     __syncthreads();
   }
 }
 
 template <int MAXE, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_fft2_cols(FftPlan F, const double2* __restrict__ tw, const double2* __restrict__ mid,
-                            double2* __restrict__ out, int32_t n1, int64_t ncols, double scale,
-                            bool inv) {
+__global__ __launch_bounds__(MAXT) void k_fft2_cols(const FftPlan* F, const double2* __restrict__ tw,
+                                                  const double2* __restrict__ mid, double2* __restrict__ out,
+                                                  int32_t n0, int32_t n1, int64_t ncols, double scale, bool inv) {
   const int T = blockDim.x, tid = threadIdx.x;
-  const int n0 = F.n;
   for (int64_t c = blockIdx.x; c < ncols; c += gridDim.x) {
     const int64_t b = c / n1, col = c % n1;
     const double2* src = mid + b * (int64_t)n0 * n1 + col;
@@ -634,20 +646,21 @@ __global__ __launch_bounds__(MAXT) void k_fft2_cols(FftPlan F, const double2* __
   }
 }
 
-int bldfm_launch_fft2(const FftPlan& F0, const double2* tw0, const FftPlan& F1, const double2* tw1,
-                      int32_t batch, const double2* in, double2* mid, double2* out, int32_t sign,
-                      double scale, int threads, int lds_bytes, int grid_cap, hipStream_t stream) {
+int bldfm_launch_fft2(const FftPlan* F0, int32_t n0, const double2* tw0, const FftPlan* F1, int32_t n1,
+                      const double2* tw1, int32_t batch, const double2* in, double2* mid, double2* out,
+                      int32_t sign, double scale, int threads, int lds_bytes, int grid_cap,
+                      hipStream_t stream) {
   const bool inv = sign > 0;
-  const int64_t nrows = (int64_t)batch * F0.n, ncols = (int64_t)batch * F1.n;
+  const int64_t nrows = (int64_t)batch * n0, ncols = (int64_t)batch * n1;
   if (threads <= 256) {
-    k_fft2_rows<FFT_SMALL_E, 256><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, nrows, inv);
+    k_fft2_rows<FFT_SMALL_E, 256><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, n1, nrows, inv);
     LAUNCH_CHECK();
-    k_fft2_cols<FFT_SMALL_E, 256><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, F1.n, ncols, scale, inv);
+    k_fft2_cols<FFT_SMALL_E, 256><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, n0, n1, ncols, scale, inv);
     LAUNCH_CHECK();
   } else {
-    k_fft2_rows<BLDFM_FFT_MAXE, 1024><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, nrows, inv);
+    k_fft2_rows<BLDFM_FFT_MAXE, 1024><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, n1, nrows, inv);
     LAUNCH_CHECK();
-    k_fft2_cols<BLDFM_FFT_MAXE, 1024><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, F1.n, ncols, scale, inv);
+    k_fft2_cols<BLDFM_FFT_MAXE, 1024><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, n0, n1, ncols, scale, inv);
     LAUNCH_CHECK();
   }
   return 0;
