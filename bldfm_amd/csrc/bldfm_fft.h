@@ -120,11 +120,15 @@ __device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const doubl
 #pragma unroll
       for (int r = 0; r < R; ++r) x[b][r] = buf[j + r * nb];
       if (k != 0) {
+        // W^{r k stride} for r = 1..R-1 from one table entry by repeated
+        // multiplication (error grows like r*eps, not amplified downstream)
+        double2 w1 = tw[k * stride];
+        if (inv) w1.y = -w1.y;
+        double2 w = w1;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          double2 w = tw[r * k * stride];
-          if (inv) w.y = -w.y;
           x[b][r] = cmul(x[b][r], w);
+          if (r + 1 < R) w = cmul(w, w1);
         }
       }
       butterfly<R>(x[b], inv);

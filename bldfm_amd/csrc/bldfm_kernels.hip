@@ -415,6 +415,22 @@ __device__ __forceinline__ void lds_zero(int n, int tid, int T) {
   __syncthreads();
 }
 
+// Padded transform input from a compact band of nb values (element stride
+// `stride` in global memory): band values at their padded indices, zeros in
+// the gap between the two halves of the band, one pass, one barrier.
+__device__ __forceinline__ void band_load(double2* buf, const double2* __restrict__ src, int64_t stride,
+                                          int nb, int n, int tid, int T) {
+  const int lo = (nb + 1) / 2;  // compact indices [0, lo) hold f >= 0 at padded [0, lo)
+  const int hi_start = n - (nb - lo);
+  for (int i = tid; i < n; i += T) {
+    double2 v = cmk(0.0, 0.0);
+    if (i < lo) v = src[(int64_t)i * stride];
+    else if (i >= hi_start) v = src[(int64_t)(i - hi_start + lo) * stride];
+    buf[i] = v;
+  }
+  __syncthreads();
+}
+
 // Forward, rows: one surface-flux row y placed at [px, px+nx) of nxe points,
 // e^{-i} DFT, scale 1/nxe (numpy norm="forward"), keep the nlx retained
 // columns in truncated order.
@@ -476,10 +492,7 @@ __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
     const InstDesc& D = P.desc[inst];
     const int64_t local = t - P.t_inv_cols[inst];  // s*nxb + cx
     const FftPlan& F = P.plans[D.plan_y];
-    lds_zero(F.n, tid, T);
-    const double2* zc = B.ws + D.off_z + local * D.nyb;
-    for (int c = tid; c < D.nyb; c += T) smem[padded_of_compact(c, D.nyb, F.n)] = zc[c];
-    __syncthreads();
+    band_load(smem, B.ws + D.off_z + local * D.nyb, 1, D.nyb, F.n, tid, T);
     bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, !D.footprint, tid, T);
     double2* v = B.ws + D.off_v + local * D.ny;
     for (int y = tid; y < D.ny; y += T) v[y] = smem[D.py + y];
@@ -499,10 +512,7 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
     const int64_t local = t - P.t_inv_rows[inst];  // s*ny + y
     const int s = (int)(local / D.ny), y = (int)(local % D.ny);
     const FftPlan& F = P.plans[D.plan_x];
-    lds_zero(F.n, tid, T);
-    const double2* v = B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y;
-    for (int c = tid; c < D.nxb; c += T) smem[padded_of_compact(c, D.nxb, F.n)] = v[(int64_t)c * D.ny];
-    __syncthreads();
+    band_load(smem, B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, D.nxb, F.n, tid, T);
     bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, !D.footprint, tid, T);
     const int64_t o = D.off_out + local * D.nx;
     if (D.out_f32) {
