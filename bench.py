@@ -156,7 +156,7 @@ def main():
         extra["fast_fma_rel_l2_vs_exact"] = d
         del fbf
 
-    cpu_fps, worst = cpu_baseline(specs, fb, args.cpu_sample)
+    cpu_fps, worst = cpu_baseline(specs, fb, args.cpu_sample) if args.cpu_sample > 0 else (None, None)
     out = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -191,7 +191,7 @@ def main():
         },
         "phase_ms_per_step": {k: round(v / max(n, 1), 4) for k, v in phases.items()},
         "cpu_baseline": {
-            "value": round(cpu_fps, 4),
+            "value": round(cpu_fps, 4) if cpu_fps is not None else None,
             "unit": "footprints/s",
             "cores": 1,
             "kind": "port",

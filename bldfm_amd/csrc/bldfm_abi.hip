@@ -373,8 +373,16 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   const size_t o_ints = take(sizeof(int32_t) * std::max<size_t>(ints.size(), 1));
   const size_t o_plans = take(sizeof(FftPlan) * plans.size());
   const size_t o_tw = take(sizeof(double2) * tw_total);
+  // tile records {instance, local index} of each kernel
+  std::vector<TileRec> recs[7];
+  for (int k = 0; k < 7; ++k) {
+    if (tiles[k][ninst] > ((int64_t)1 << 31)) return bad(BLDFM_ELIMIT, "too many tiles in one plan");
+    recs[k].resize((size_t)tiles[k][ninst]);
+    for (int i = 0; i < ninst; ++i)
+      for (int64_t t = tiles[k][i]; t < tiles[k][i + 1]; ++t) recs[k][t] = TileRec{i, (int32_t)(t - tiles[k][i])};
+  }
   size_t o_tiles[7];
-  for (int k = 0; k < 7; ++k) o_tiles[k] = take(sizeof(int64_t) * (ninst + 1));
+  for (int k = 0; k < 7; ++k) o_tiles[k] = take(sizeof(TileRec) * std::max<size_t>(recs[k].size(), 1));
   std::vector<char> blob(off, 0);
   memcpy(blob.data() + o_desc, P->desc.data(), sizeof(InstDesc) * ninst);
   memcpy(blob.data() + o_met, met.data(), sizeof(ProfSetup) * ninst);
@@ -382,7 +390,8 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   if (!ints.empty()) memcpy(blob.data() + o_ints, ints.data(), sizeof(int32_t) * ints.size());
   memcpy(blob.data() + o_plans, plans.data(), sizeof(FftPlan) * plans.size());
   memcpy(blob.data() + o_tw, tw.data(), sizeof(double2) * tw_total);
-  for (int k = 0; k < 7; ++k) memcpy(blob.data() + o_tiles[k], tiles[k].data(), sizeof(int64_t) * (ninst + 1));
+  for (int k = 0; k < 7; ++k)
+    if (!recs[k].empty()) memcpy(blob.data() + o_tiles[k], recs[k].data(), sizeof(TileRec) * recs[k].size());
   hipError_t e = hipMalloc(&P->dmeta, off);
   if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(plan metadata)"); }
   e = hipMemcpy(P->dmeta, blob.data(), off, hipMemcpyHostToDevice);
@@ -396,13 +405,13 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   V.ints = (const int32_t*)(base + o_ints);
   V.plans = (const FftPlan*)(base + o_plans);
   V.tw = (const double2*)(base + o_tw);
-  V.t_tab = (const int64_t*)(base + o_tiles[0]);
-  V.t_fwd_rows = (const int64_t*)(base + o_tiles[1]);
-  V.t_fwd_cols = (const int64_t*)(base + o_tiles[2]);
-  V.t_shoot = (const int64_t*)(base + o_tiles[3]);
-  V.t_nyq = (const int64_t*)(base + o_tiles[4]);
-  V.t_inv_cols = (const int64_t*)(base + o_tiles[5]);
-  V.t_inv_rows = (const int64_t*)(base + o_tiles[6]);
+  V.t_tab = (const TileRec*)(base + o_tiles[0]);
+  V.t_fwd_rows = (const TileRec*)(base + o_tiles[1]);
+  V.t_fwd_cols = (const TileRec*)(base + o_tiles[2]);
+  V.t_shoot = (const TileRec*)(base + o_tiles[3]);
+  V.t_nyq = (const TileRec*)(base + o_tiles[4]);
+  V.t_inv_cols = (const TileRec*)(base + o_tiles[5]);
+  V.t_inv_rows = (const TileRec*)(base + o_tiles[6]);
   V.ninst = ninst;
   V.any_dispersion = any_disp;
   for (int k = 0; k < 4; ++k)

@@ -13,14 +13,14 @@ struct PlanDev {
   const int32_t* ints;        // slot maps and level indices
   const FftPlan* plans;       // FFT plans
   const double2* tw;          // twiddle tables
-  // tile prefix sums (ninst + 1 each)
-  const int64_t* t_tab;
-  const int64_t* t_fwd_rows;
-  const int64_t* t_fwd_cols;
-  const int64_t* t_shoot;
-  const int64_t* t_nyq;
-  const int64_t* t_inv_cols;
-  const int64_t* t_inv_rows;
+  // per-kernel tile records {instance, local tile index}
+  const TileRec* t_tab;
+  const TileRec* t_fwd_rows;
+  const TileRec* t_fwd_cols;
+  const TileRec* t_shoot;
+  const TileRec* t_nyq;
+  const TileRec* t_inv_cols;
+  const TileRec* t_inv_rows;
   int32_t ninst;
   int32_t any_dispersion;
   int32_t variant_used[4][2]; // shooting kernels this plan launches [variant][exact]

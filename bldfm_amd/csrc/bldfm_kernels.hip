@@ -28,13 +28,18 @@ __device__ __forceinline__ int64_t xcd_swizzle(int64_t b, int64_t n) {
   return x * q + (x < r ? x : r) + i;
 }
 
-__device__ __forceinline__ int find_inst(const int64_t* __restrict__ off, int ninst, int64_t t) {
-  int lo = 0, hi = ninst - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= t) lo = mid; else hi = mid - 1;
-  }
-  return lo;
+// Uniform per-tile data read through the constant address space (scalar
+// loads into SGPRs): tile records, instance descriptors, FFT plans.
+typedef const __attribute__((address_space(4))) InstDesc cdesc_t;
+typedef const __attribute__((address_space(4))) FftPlan cplan_t;
+typedef const __attribute__((address_space(4))) TileRec ctile_t;
+
+__device__ __forceinline__ TileRec tile_rec(const TileRec* recs, int64_t t) {
+  const ctile_t* r = (const ctile_t*)(recs + t);
+  TileRec o;
+  o.inst = r->inst;
+  o.local = r->local;
+  return o;
 }
 
 // ---------------------------------------------------------------- level table
@@ -53,10 +58,11 @@ __device__ __forceinline__ int find_inst(const int64_t* __restrict__ off, int ni
 __global__ __launch_bounds__(256) void k_level_table(PlanDev P, ExecBufs B) {
 #pragma clang fp contract(off)
   for (int64_t t = blockIdx.x; t < P.tot_tab; t += gridDim.x) {
-    const int inst = find_inst(P.t_tab, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
+    const TileRec tr = tile_rec(P.t_tab, t);
+    const int inst = tr.inst;
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + inst);
     const int nz = D.nz;
-    const int i = (int)(t - P.t_tab[inst]) * blockDim.x + threadIdx.x;
+    const int i = tr.local * (int)blockDim.x + (int)threadIdx.x;
     if (i >= nz) continue;
     double* prof = B.prof + D.off_prof;
     double cur[6], nxt[6];
@@ -113,7 +119,7 @@ struct Lane {
   double lx, ly;
 };
 
-__device__ __forceinline__ bool lane_setup(const InstDesc& D, int lane, Lane& L) {
+__device__ __forceinline__ bool lane_setup(const cdesc_t& D, int lane, Lane& L) {
   if (lane >= D.lanes) return false;
   L.fx = 0; L.fy = 0; L.nidx = -1;
   L.kind = decode_lane(lane, D.nlx, D.nly, &L.fx, &L.fy, &L.nidx);
@@ -125,7 +131,7 @@ __device__ __forceinline__ bool lane_setup(const InstDesc& D, int lane, Lane& L)
 }
 
 // Packed output of one mode (after complex64 rounding and phase shift).
-__device__ __forceinline__ void emit_mode(const InstDesc& D, const Lane& L, double2* __restrict__ ws,
+__device__ __forceinline__ void emit_mode(const cdesc_t& D, const Lane& L, double2* __restrict__ ws,
                                           int s, double2 Xp, double2 Xq, double2 sh) {
   if (D.single) { Xp = round_c64(Xp); Xq = round_c64(Xq); }
   if (D.has_shift) { Xp = ncmul(Xp, sh); Xq = ncmul(Xq, sh); }
@@ -143,14 +149,14 @@ __device__ __forceinline__ void emit_mode(const InstDesc& D, const Lane& L, doub
   }
 }
 
-__device__ __forceinline__ void emit_origin(const InstDesc& D, double2* __restrict__ ws, int s,
+__device__ __forceinline__ void emit_origin(const cdesc_t& D, double2* __restrict__ ws, int s,
                                             double p00, double q00) {
   if (D.single) { p00 = (double)(float)p00; q00 = (double)(float)q00; }
   ws[D.off_z + (int64_t)s * D.nxb * D.nyb] = cmk(p00, q00);
 }
 
 // e^{i(lx*shx + ly*shy)} (src/bldfm/solver.py:254-262)
-__device__ __forceinline__ double2 phase_shift(const InstDesc& D, const Lane& L) {
+__device__ __forceinline__ double2 phase_shift(const cdesc_t& D, const Lane& L) {
   double s = 0.0, c = 1.0;
   if (D.has_shift) {
     const double th = xmul(L.lx, D.shx) + xmul(L.ly, D.shy);
@@ -159,7 +165,7 @@ __device__ __forceinline__ double2 phase_shift(const InstDesc& D, const Lane& L)
   return cmk(c, s);
 }
 
-__device__ __forceinline__ double2 ivp2_init(const InstDesc& D, const Lane& L,
+__device__ __forceinline__ double2 ivp2_init(const cdesc_t& D, const Lane& L,
                                              const double2* __restrict__ ws) {
   if (D.footprint) return cmk(D.q00, 0.0);
   const int jx = trunc_of_freq(L.fx, D.nlx), jy = trunc_of_freq(L.fy, D.nly);
@@ -252,7 +258,7 @@ __device__ __forceinline__ double2 shoot_alpha(const double* __restrict__ tt, do
 
 // Degenerate mode (0,0): flux conserved, p by the trapezoid rule in z
 // (src/bldfm/solver.py:190-191, 239-251): p00 <- p00 - (q00*dz)*W.
-__device__ void origin_numeric(const InstDesc& D, const double* __restrict__ tab,
+__device__ void origin_numeric(const cdesc_t& D, const double* __restrict__ tab,
                                const int32_t* __restrict__ slot, double2* __restrict__ ws) {
 #pragma clang fp contract(off)
   const double q00 = D.footprint ? D.q00 : ws[D.off_q0].x;
@@ -274,11 +280,11 @@ __device__ void origin_numeric(const InstDesc& D, const double* __restrict__ tab
 template <int MAXS, bool RECOMP, bool EXACT>
 __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, int variant) {
   for (int64_t t = blockIdx.x; t < P.tot_shoot; t += gridDim.x) {
-    const int inst = find_inst(P.t_shoot, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
+    const TileRec tr = tile_rec(P.t_shoot, t);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     if (D.variant != variant || D.exact != (int)EXACT) continue;
     Lane L;
-    const int lane = (int)(t - P.t_shoot[inst]) * SHOOT_THREADS + threadIdx.x;
+    const int lane = tr.local * SHOOT_THREADS + (int)threadIdx.x;
     if (!lane_setup(D, lane, L)) continue;
     const double* __restrict__ tab = B.tab + D.off_tab;
     ctab_t* __restrict__ ctab = (ctab_t*)tab;
@@ -349,11 +355,11 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
 __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot_analytic(PlanDev P, ExecBufs B, int variant) {
 #pragma clang fp contract(off)
   for (int64_t t = blockIdx.x; t < P.tot_shoot; t += gridDim.x) {
-    const int inst = find_inst(P.t_shoot, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
+    const TileRec tr = tile_rec(P.t_shoot, t);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     if (D.variant != variant) continue;
     Lane L;
-    const int lane = (int)(t - P.t_shoot[inst]) * SHOOT_THREADS + threadIdx.x;
+    const int lane = tr.local * SHOOT_THREADS + (int)threadIdx.x;
     if (!lane_setup(D, lane, L)) continue;
     const int nz = D.nz;
     const double* tt = B.tab + D.off_tab + (int64_t)(nz - 1) * BLDFM_LEVEL_STRIDE;
@@ -386,11 +392,11 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot_analytic(PlanDev P, Exe
 // Z(c) = (X(c) + conj X(-c))/2 packed as Hp + i Hq.
 __global__ __launch_bounds__(256) void k_nyq_fix(PlanDev P, ExecBufs B) {
   for (int64_t t = blockIdx.x; t < P.tot_nyq; t += gridDim.x) {
-    const int inst = find_inst(P.t_nyq, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
+    const TileRec tr = tile_rec(P.t_nyq, t);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     const int cnt = nyq_fix_count(D.nlx, D.nly, D.nxb, D.nyb);
     const int bps = (cnt + 255) / 256;
-    const int local = (int)(t - P.t_nyq[inst]);
+    const int local = tr.local;
     const int s = local / bps;
     const int p = (local % bps) * 256 + threadIdx.x;
     if (p >= cnt) continue;
@@ -439,15 +445,15 @@ __global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t b = blockIdx.x; b < P.tot_fwd_rows; b += gridDim.x) {
     const int64_t t = gridDim.x >= P.tot_fwd_rows ? xcd_swizzle(b, P.tot_fwd_rows) : b;
-    const int inst = find_inst(P.t_fwd_rows, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
-    const int y = (int)(t - P.t_fwd_rows[inst]);
-    const FftPlan& F = P.plans[D.plan_x];
+    const TileRec tr = tile_rec(P.t_fwd_rows, t);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
+    const int y = tr.local;
+    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
     lds_zero(F.n, tid, T);
     const double* row = B.src + D.off_src + (int64_t)y * D.nx;
     for (int x = tid; x < D.nx; x += T) smem[D.px + x] = cmk(row[x], 0.0);
     __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, false, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, false, tid, T);
     const double fct = 1.0 / (double)F.n;
     double2* aw = B.ws + D.off_aw + (int64_t)y * D.nlx;
     for (int j = tid; j < D.nlx; j += T)
@@ -463,15 +469,15 @@ __global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t b = blockIdx.x; b < P.tot_fwd_cols; b += gridDim.x) {
     const int64_t t = gridDim.x >= P.tot_fwd_cols ? xcd_swizzle(b, P.tot_fwd_cols) : b;
-    const int inst = find_inst(P.t_fwd_cols, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
-    const int jx = (int)(t - P.t_fwd_cols[inst]);
-    const FftPlan& F = P.plans[D.plan_y];
+    const TileRec tr = tile_rec(P.t_fwd_cols, t);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
+    const int jx = tr.local;
+    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
     lds_zero(F.n, tid, T);
     const double2* aw = B.ws + D.off_aw;
     for (int y = tid; y < D.ny; y += T) smem[D.py + y] = aw[(int64_t)y * D.nlx + jx];
     __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, false, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, false, tid, T);
     const double fct = 1.0 / (double)F.n;
     double2* q0 = B.ws + D.off_q0 + (int64_t)jx * D.nly;
     for (int j = tid; j < D.nly; j += T)
@@ -488,12 +494,12 @@ __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t b = blockIdx.x; b < P.tot_inv_cols; b += gridDim.x) {
     const int64_t t = gridDim.x >= P.tot_inv_cols ? xcd_swizzle(b, P.tot_inv_cols) : b;
-    const int inst = find_inst(P.t_inv_cols, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
-    const int64_t local = t - P.t_inv_cols[inst];  // s*nxb + cx
-    const FftPlan& F = P.plans[D.plan_y];
+    const TileRec tr = tile_rec(P.t_inv_cols, t);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
+    const int64_t local = tr.local;  // s*nxb + cx
+    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
     band_load(smem, B.ws + D.off_z + local * D.nyb, 1, D.nyb, F.n, tid, T);
-    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, !D.footprint, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, !D.footprint, tid, T);
     double2* v = B.ws + D.off_v + local * D.ny;
     for (int y = tid; y < D.ny; y += T) v[y] = smem[D.py + y];
     __syncthreads();
@@ -507,13 +513,13 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t b = blockIdx.x; b < P.tot_inv_rows; b += gridDim.x) {
     const int64_t t = gridDim.x >= P.tot_inv_rows ? xcd_swizzle(b, P.tot_inv_rows) : b;
-    const int inst = find_inst(P.t_inv_rows, P.ninst, t);
-    const InstDesc& D = P.desc[inst];
-    const int64_t local = t - P.t_inv_rows[inst];  // s*ny + y
+    const TileRec tr = tile_rec(P.t_inv_rows, t);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
+    const int64_t local = tr.local;  // s*ny + y
     const int s = (int)(local / D.ny), y = (int)(local % D.ny);
-    const FftPlan& F = P.plans[D.plan_x];
+    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
     band_load(smem, B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, D.nxb, F.n, tid, T);
-    bldfm_fft::lds_fft<MAXE>(smem, &F, P.tw + F.tw_off, !D.footprint, tid, T);
+    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, !D.footprint, tid, T);
     const int64_t o = D.off_out + local * D.nx;
     if (D.out_f32) {
       float* conc = (float*)B.conc + o;

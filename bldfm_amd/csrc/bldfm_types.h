@@ -37,6 +37,12 @@ struct FftPlan {
   int64_t tw_off;  // offset (in double2) of the n twiddles e^{-2 pi i t/n}
 };
 
+// One tile of a kernel's work: the instance and the tile's index inside it.
+struct TileRec {
+  int32_t inst;
+  int32_t local;
+};
+
 // Mode kinds produced by the lane decoder.
 enum LaneKind : int32_t {
   LANE_IDLE = 0,
