@@ -65,6 +65,19 @@ def workload(batch, seed=20261204):
     return specs
 
 
+def measured_traffic(kernel="k_shoot"):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", "footprint_batch_512x64_pmc.txt")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def timed(batch_obj, steps, warmup):
     import torch
 
@@ -183,7 +196,9 @@ def main():
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64, 4),
-            "traffic": None,
+            "traffic": measured_traffic() if args.batch == 128 else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated; profiles/)",
+            "algorithmic_bytes": 16 * sum(2 * (l - 1) for l, _ in fb.lanes_and_steps()),
             "flop_per_unit": FLOP_PER_MODE_STEP,
             "units_per_launch": units,
             "avg_launch_ms": round(shoot_ms, 4),
