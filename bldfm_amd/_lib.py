@@ -58,13 +58,33 @@ class Instance(ctypes.Structure):
         ("has_shift", ctypes.c_int32),
         ("prof_from_met", ctypes.c_int32),
         ("fast", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("surrogate", ctypes.c_int32),
         ("dx", ctypes.c_double),
         ("dy", ctypes.c_double),
         ("shift_x", ctypes.c_double),
         ("shift_y", ctypes.c_double),
         ("srf_bg_conc", ctypes.c_double),
         ("met", Met),
+    ]
+
+
+class SurHull(ctypes.Structure):
+    """Mirror of ``bldfm_sur_hull``."""
+
+    _fields_ = [
+        ("ustar_lo", ctypes.c_double),
+        ("ustar_hi", ctypes.c_double),
+        ("invl_lo", ctypes.c_double),
+        ("invl_hi", ctypes.c_double),
+        ("z0_lo", ctypes.c_double),
+        ("z0_hi", ctypes.c_double),
+        ("z0_zm_max", ctypes.c_double),
+        ("zm_lo", ctypes.c_double),
+        ("zm_hi", ctypes.c_double),
+        ("kmax", ctypes.c_double),
+        ("top_ratio", ctypes.c_double),
+        ("n", ctypes.c_int32),
+        ("closure", ctypes.c_int32),
     ]
 
 
@@ -91,6 +111,16 @@ SIGNATURES = {
     "bldfm_math_probe": (ctypes.c_int, [_I32, _P, _P, _I64, _P, _P]),
     "bldfm_fft2_workspace_bytes": (ctypes.c_size_t, [_I32, _I32, _I32]),
     "bldfm_fft2": (ctypes.c_int, [ctypes.c_int, _I32, _I32, _I32, _P, _P, _I32, _D, _P, _P]),
+    "bldfm_sur_sizes": (ctypes.c_int, [ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
+    "bldfm_sur_met": (ctypes.c_int, [ctypes.POINTER(Met), _I32, _P]),
+    "bldfm_transfer_tables": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Met), _I32, _I32, _P, _P, _P]),
+    "bldfm_transfer_solve": (ctypes.c_int, [_P, _I32, _P, _P, _I32, _P, _I32, _P, _P, _I32, _P]),
+    "bldfm_sur_pack": (ctypes.c_int, [_P, _P, _P]),
+    "bldfm_sur_forward": (ctypes.c_int, [_P, _P, _I32, _P, _P, _I64, _P, _P]),
+    "bldfm_sur_train_grad": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P]),
+    "bldfm_sur_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I32, _P]),
+    "bldfm_plan_set_surrogate": (ctypes.c_int, [_P, _P, ctypes.POINTER(SurHull), _D]),
+    "bldfm_plan_sur_stats": (ctypes.c_int, [_P, _P, _P, _P, _P]),
 }
 
 _lock = threading.Lock()
@@ -217,6 +247,20 @@ class Plan:
         n = _I32()
         check(load().bldfm_plan_timing_end(self.handle, ms, ctypes.byref(n)), "timing_end")
         return dict(zip(self.PHASES, list(ms))), n.value
+
+    def set_surrogate(self, net, hull, tol):
+        """Serve surrogate-flagged instances with the network buffer ``net``
+        (a device tensor, or None to switch the surrogate off)."""
+        check(load().bldfm_plan_set_surrogate(self.handle, ptr(net), ctypes.byref(hull) if hull is not None else None,
+                                              float(tol)), "plan_set_surrogate")
+
+    def surrogate_stats(self, workspace, n):
+        """Per-instance spot-check error (NaN if not served) and path (0 surrogate, 1 solver)."""
+        err = (ctypes.c_double * n)()
+        mode = (ctypes.c_int32 * n)()
+        check(load().bldfm_plan_sur_stats(self.handle, ptr(workspace), err, mode, stream_handle(self.device)),
+              "plan_sur_stats")
+        return list(err), list(mode)
 
     def __del__(self):
         h = getattr(self, "handle", None)

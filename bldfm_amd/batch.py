@@ -56,7 +56,7 @@ class FootprintSpec:
     extra: dict = field(default_factory=dict)
 
 
-def _instance(spec: FootprintSpec, precision: str, fast: bool):
+def _instance(spec: FootprintSpec, precision: str, fast: bool, surrogate: bool = False):
     xmx, ymx = spec.domain
     nlx, nly = spec.modes
     if (nlx % 2 > 0) or (nly % 2 > 0):
@@ -95,6 +95,7 @@ def _instance(spec: FootprintSpec, precision: str, fast: bool):
     inst.has_shift = 1
     inst.prof_from_met = 1
     inst.fast = bool(fast)
+    inst.surrogate = bool(surrogate)
     inst.dx, inst.dy = dx, dy
     inst.shift_x, inst.shift_y = spec.meas_pt[0] + halo, spec.meas_pt[1] + halo
     inst.srf_bg_conc = 0.0
@@ -110,14 +111,21 @@ def _instance(spec: FootprintSpec, precision: str, fast: bool):
 
 
 class FootprintBatch:
-    """A batch of footprint solves sharing one plan and one set of buffers."""
+    """A batch of footprint solves sharing one plan and one set of buffers.
 
-    def __init__(self, specs: Sequence[FootprintSpec], precision: str = "double", fast: bool = False):
+    With ``surrogate`` (a trained ``bldfm_amd.surrogate.SurrogateNet``), each
+    execution serves the instances inside the network's training hull from
+    the network, re-solves one mode in 100 of each in FP64, and solves an
+    instance in FP64 when that spot check's relative L2 error exceeds
+    ``tol``.  Served instances produce the flux footprint (conc is 0)."""
+
+    def __init__(self, specs: Sequence[FootprintSpec], precision: str = "double", fast: bool = False,
+                 surrogate=None, tol: float = 1e-3):
         if len(specs) == 0:
             raise ValueError("empty batch")
         self.specs = list(specs)
         self.device = _lib.device()
-        built = [_instance(s, precision, fast) for s in self.specs]
+        built = [_instance(s, precision, fast, surrogate is not None) for s in self.specs]
         self.instances = [b[0] for b in built]
         ints = np.concatenate([b[1] for b in built]).astype(np.int32)
         self.plan = _lib.Plan(self.instances, ints, self.device)
@@ -127,6 +135,14 @@ class FootprintBatch:
         self.flx = torch.empty_like(self.conc)
         self.workspace = torch.empty(max(self.plan.workspace_bytes, 1), dtype=torch.uint8, device=dev)
         self._graph = None
+        self.surrogate = surrogate
+        if surrogate is not None:
+            self.plan.set_surrogate(surrogate.net, surrogate.hull.c_struct(), tol)
+
+    def surrogate_stats(self):
+        """After a run: per instance the spot-check relative L2 error (NaN
+        when outside the hull) and the path taken (0 surrogate, 1 solver)."""
+        return self.plan.surrogate_stats(self.workspace, len(self.specs))
 
     def __len__(self):
         return len(self.specs)

@@ -164,7 +164,48 @@ struct bldfm_plan {
   // phase timing (bldfm_plan_timing_begin/end): 7 events per execution
   std::vector<hipEvent_t> events;
   int timing_max = 0, timing_count = 0;
+  // surrogate: instance arguments and met setups for the hull check, gate
+  // state offsets (double2 units into the complex workspace)
+  std::vector<bldfm_instance> inst;
+  std::vector<ProfSetup> setups;
+  int sur_any = 0;
+  int64_t off_gate = 0, off_mode = 0;
+  void* sur_ok_dev = nullptr;
 };
+
+// Network inputs of one met setup (layout of SurMet); n is the level index
+// of the measurement height.
+static SurMet sur_met_of(const ProfSetup& s, int n) {
+  SurMet m;
+  memset(&m, 0, sizeof(m));
+  const double il = 1.0 / s.mol;
+  double dir = atan2(-s.um, -s.vm);  // compute_wind_fields: u = -U sin(dir), v = -U cos(dir)
+  if (dir < 0.0) dir += 2.0 * BLDFM_PI;
+  const double wx = s.um / s.absum, wy = s.vm / s.absum;
+  m.met[0] = (float)s.ustar;
+  m.met[1] = (float)il;
+  m.met[2] = (float)log(s.z0);
+  m.met[3] = (float)s.zm;
+  m.met[4] = (float)s.absum;
+  m.met[5] = (float)dir;
+  m.met[6] = (float)(s.zm * il);
+  m.met[7] = (float)log(s.z0 / s.zm);
+  m.wx = (float)wx;
+  m.wy = (float)wy;
+  m.zm = (float)s.zm;
+  // WKB table: interval heights and along-wind speed over diffusivity at the midpoints
+  const int nw = std::max(0, std::min(std::min(n, s.nlev - 1), SUR_NWKB));
+  m.nw = (float)nw;
+  double a[6], b[6];
+  for (int j = 0; j < nw; ++j) {
+    prof_point(s, j, a);
+    prof_point(s, j + 1, b);
+    const double ua = a[1] * wx + a[2] * wy, ub = b[1] * wx + b[2] * wy;
+    m.wkb[j][0] = (float)(b[0] - a[0]);
+    m.wkb[j][1] = (float)((ua + ub) / (a[5] + b[5]));
+  }
+  return m;
+}
 
 static void destroy_events(bldfm_plan* P) {
   for (hipEvent_t e : P->events) (void)hipEventDestroy(e);
@@ -235,8 +276,11 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
     plan_n.push_back(n);
     return (int)plan_n.size() - 1;
   };
-  std::vector<int64_t> tiles[7];
+  constexpr int kKinds = 9;  // tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows, sur, spot
+  std::vector<int64_t> tiles[kKinds];
   for (auto& v : tiles) v.assign(ninst + 1, 0);
+  std::vector<SurMet> surmet(ninst);
+  P->inst.assign(inst, inst + ninst);
   int64_t off_prof = 0, off_tab = 0, off_src = 0, off_ws = 0, off_out = 0;
   const int32_t* lv = levels;
   int out_f32 = -1, any_disp = 0, variant_used[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -321,6 +365,13 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
     D.variant = D.analytic ? 3 : (nfilled <= 1 ? 0 : (nfilled <= 4 ? 1 : 2));
     D.exact = I.fast ? 0 : 1;
     variant_used[D.variant][D.exact] = 1;
+    const bool want_sur = I.surrogate != 0 && I.prof_from_met && D.footprint;
+    if (want_sur) {
+      surmet[i] = sur_met_of(met[i], I.met.n);
+      P->sur_any = 1;
+    } else {
+      memset(&surmet[i], 0, sizeof(SurMet));
+    }
 
     // buffer offsets
     D.off_prof = off_prof; off_prof += 6LL * D.nz;
@@ -340,7 +391,8 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
 
     // tiles: tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows
     const int fixc = nyq_fix_count(D.nlx, D.nly, D.nxb, D.nyb);
-    const int64_t nt[7] = {
+    const int64_t nspot = (D.lanes + BLDFM_SUR_SPOT_STRIDE - 1) / BLDFM_SUR_SPOT_STRIDE;
+    const int64_t nt[kKinds] = {
         (D.nz + 255) / 256,
         D.footprint ? 0 : D.ny,
         D.footprint ? 0 : D.nlx,
@@ -348,10 +400,19 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
         fixc > 0 ? (int64_t)D.nslot * ((fixc + 255) / 256) : 0,
         (int64_t)D.nslot * D.nxb,
         (int64_t)D.nslot * D.ny,
+        want_sur ? (D.lanes + SUR_PTS - 1) / SUR_PTS : 0,
+        want_sur ? (nspot + 255) / 256 : 0,
     };
-    for (int k = 0; k < 7; ++k) tiles[k][i + 1] = tiles[k][i] + nt[k];
+    for (int k = 0; k < kKinds; ++k) tiles[k][i + 1] = tiles[k][i] + nt[k];
   }
   if (lv - levels > (int64_t)1 << 31) return bad(BLDFM_EINVAL, "level table too large");
+  P->setups = met;
+  if (P->sur_any) {  // gate sums (double2) and modes (int32) per instance
+    P->off_gate = off_ws;
+    off_ws += ninst;
+    P->off_mode = off_ws;
+    off_ws += (ninst + 3) / 4;
+  }
 
   // FFT plans and twiddles
   std::vector<FftPlan> plans(plan_n.size());
@@ -373,16 +434,17 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   const size_t o_ints = take(sizeof(int32_t) * std::max<size_t>(ints.size(), 1));
   const size_t o_plans = take(sizeof(FftPlan) * plans.size());
   const size_t o_tw = take(sizeof(double2) * tw_total);
+  const size_t o_surmet = take(sizeof(SurMet) * ninst);
   // tile records {instance, local index} of each kernel
-  std::vector<TileRec> recs[7];
-  for (int k = 0; k < 7; ++k) {
+  std::vector<TileRec> recs[kKinds];
+  for (int k = 0; k < kKinds; ++k) {
     if (tiles[k][ninst] > ((int64_t)1 << 31)) return bad(BLDFM_ELIMIT, "too many tiles in one plan");
     recs[k].resize((size_t)tiles[k][ninst]);
     for (int i = 0; i < ninst; ++i)
       for (int64_t t = tiles[k][i]; t < tiles[k][i + 1]; ++t) recs[k][t] = TileRec{i, (int32_t)(t - tiles[k][i])};
   }
-  size_t o_tiles[7];
-  for (int k = 0; k < 7; ++k) o_tiles[k] = take(sizeof(TileRec) * std::max<size_t>(recs[k].size(), 1));
+  size_t o_tiles[kKinds];
+  for (int k = 0; k < kKinds; ++k) o_tiles[k] = take(sizeof(TileRec) * std::max<size_t>(recs[k].size(), 1));
   std::vector<char> blob(off, 0);
   memcpy(blob.data() + o_desc, P->desc.data(), sizeof(InstDesc) * ninst);
   memcpy(blob.data() + o_met, met.data(), sizeof(ProfSetup) * ninst);
@@ -390,7 +452,8 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   if (!ints.empty()) memcpy(blob.data() + o_ints, ints.data(), sizeof(int32_t) * ints.size());
   memcpy(blob.data() + o_plans, plans.data(), sizeof(FftPlan) * plans.size());
   memcpy(blob.data() + o_tw, tw.data(), sizeof(double2) * tw_total);
-  for (int k = 0; k < 7; ++k)
+  memcpy(blob.data() + o_surmet, surmet.data(), sizeof(SurMet) * ninst);
+  for (int k = 0; k < kKinds; ++k)
     if (!recs[k].empty()) memcpy(blob.data() + o_tiles[k], recs[k].data(), sizeof(TileRec) * recs[k].size());
   hipError_t e = hipMalloc(&P->dmeta, off);
   if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(plan metadata)"); }
@@ -412,6 +475,15 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   V.t_nyq = (const TileRec*)(base + o_tiles[4]);
   V.t_inv_cols = (const TileRec*)(base + o_tiles[5]);
   V.t_inv_rows = (const TileRec*)(base + o_tiles[6]);
+  V.t_sur = (const TileRec*)(base + o_tiles[7]);
+  V.t_spot = (const TileRec*)(base + o_tiles[8]);
+  V.sur_met = (const SurMet*)(base + o_surmet);
+  V.sur_ok = nullptr;
+  V.sur_net = nullptr;
+  V.sur_tol = 0.0;
+  V.sur_stride = BLDFM_SUR_SPOT_STRIDE;
+  V.tot_sur = tiles[7][ninst];
+  V.tot_spot = tiles[8][ninst];
   V.ninst = ninst;
   V.any_dispersion = any_disp;
   for (int k = 0; k < 4; ++k)
@@ -440,10 +512,11 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
 
 int bldfm_plan_destroy(bldfm_plan* plan) {
   if (!plan) return 0;
-  if (plan->dmeta || !plan->events.empty()) {
+  if (plan->dmeta || plan->sur_ok_dev || !plan->events.empty()) {
     DeviceGuard g(plan->device);
     destroy_events(plan);
     if (plan->dmeta) (void)hipFree(plan->dmeta);
+    if (plan->sur_ok_dev) (void)hipFree(plan->sur_ok_dev);
   }
   delete plan;
   return 0;
@@ -483,6 +556,9 @@ int bldfm_plan_execute(bldfm_plan* P, double* prof, const double* src, void* con
   B.flx = flx;
   B.tab = (double*)workspace;
   B.ws = (double2*)((char*)workspace + align_up(P->tab_doubles * 8, 256));
+  const bool sur = P->sur_any && P->dev.sur_net != nullptr;
+  B.gate = sur ? B.ws + P->off_gate : nullptr;
+  B.mode = sur ? (int32_t*)(B.ws + P->off_mode) : nullptr;
   hipEvent_t* ev = nullptr;
   if (P->timing_count < P->timing_max) ev = P->events.data() + 7 * (P->timing_count++);
   const int rc = bldfm_launch_all(P->dev, B, kGridCap, stream, ev);
@@ -580,6 +656,194 @@ int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* 
                          (double2*)out, sign, scale, fft_threads_for(nmax), nmax * 16, kGridCap, stream);
   if (rc) return fail(BLDFM_EHIP, std::string("fft2 launch failed: ") +
                                       hipGetErrorString((hipError_t)(-(rc + 1000))));
+  return 0;
+}
+
+// ------------------------------------------------------------------ surrogate
+static int launch_fail(int rc, const char* what) {
+  return fail(BLDFM_EHIP, std::string(what) + " launch failed: " +
+                              hipGetErrorString((hipError_t)(-(rc + 1000))));
+}
+
+static constexpr int64_t kSurTrainBytesPerPoint =
+    (int64_t)(SUR_IN + SUR_NL * SUR_W) * 2 + 2LL * SUR_NL * SUR_W * 2 + 8;
+
+int bldfm_sur_sizes(int64_t* nparam, int64_t* net_bytes, int64_t* train_bytes_per_point,
+                    int32_t* block_points) {
+  if (nparam) *nparam = SUR_NPARAM;
+  if (net_bytes) *net_bytes = SUR_NET_BYTES;
+  if (train_bytes_per_point) *train_bytes_per_point = kSurTrainBytesPerPoint;
+  if (block_points) *block_points = SUR_PTS;
+  return 0;
+}
+
+int bldfm_sur_met(const bldfm_met* met, int32_t nmet, float* surmet) {
+  if (!met || nmet <= 0 || !surmet) return fail(BLDFM_EINVAL, "bad sur_met arguments");
+  for (int i = 0; i < nmet; ++i) {
+    ProfSetup s;
+    const int rc = met_setup(met + i, &s);
+    if (rc) return rc;
+    const SurMet m = sur_met_of(s, met[i].n);
+    memcpy(surmet + (size_t)i * (sizeof(SurMet) / sizeof(float)), &m, sizeof(SurMet));
+  }
+  return 0;
+}
+
+int bldfm_transfer_tables(int device, const bldfm_met* met, int32_t nmet, int32_t rows, double* tab,
+                          int32_t* nz, hipStream_t stream) {
+  if (!met || nmet <= 0 || rows < 2 || !tab || !nz) return fail(BLDFM_EINVAL, "bad transfer_tables arguments");
+  std::vector<ProfSetup> S(nmet);
+  for (int i = 0; i < nmet; ++i) {
+    const int rc = met_setup(met + i, &S[i]);
+    if (rc) return rc;
+    if (S[i].nlev > rows)
+      return fail(BLDFM_EINVAL, "met sample " + std::to_string(i) + " has more levels than rows");
+    nz[i] = S[i].nlev;
+  }
+  DeviceGuard guard(device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(ProfSetup) * nmet);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(met setups)");
+  e = hipMemcpy(d, S.data(), sizeof(ProfSetup) * nmet, hipMemcpyHostToDevice);
+  int rc = 0;
+  if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(met setups)");
+  if (!rc) {
+    const int lr = bldfm_launch_tables((const ProfSetup*)d, nmet, rows, tab, stream);
+    if (lr) rc = launch_fail(lr, "level table");
+  }
+  if (!rc) {
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int bldfm_transfer_solve(const double* tab, int32_t rows, const int32_t* nz, const int32_t* lvl,
+                         int32_t nmet, const int32_t* blk_met, int32_t nblk, const double* kxy,
+                         double* g, int32_t exact, hipStream_t stream) {
+  if (!tab || !nz || !lvl || !blk_met || !kxy || !g || rows < 2 || nmet <= 0 || nblk <= 0)
+    return fail(BLDFM_EINVAL, "bad transfer_solve arguments");
+  const int rc = bldfm_launch_transfer(tab, rows, nz, lvl, nmet, blk_met, nblk, (const double2*)kxy,
+                                       (double2*)g, exact ? 1 : 0, stream);
+  return rc ? launch_fail(rc, "transfer") : 0;
+}
+
+int bldfm_sur_pack(const float* params, void* net, hipStream_t stream) {
+  if (!params || !net) return fail(BLDFM_EINVAL, "bad sur_pack arguments");
+  const int rc = bldfm_launch_sur_adam(const_cast<float*>(params), nullptr, nullptr, nullptr, (uint8_t*)net, 0,
+                                       0.f, 0.f, 0.f, 0.f, 0.f, 0.f, stream);
+  return rc ? launch_fail(rc, "surrogate pack") : 0;
+}
+
+static int sur_points_ok(int64_t npts) {
+  if (npts <= 0 || npts % SUR_PTS) return fail(BLDFM_EINVAL, "npts must be a positive multiple of the block size");
+  if (npts / SUR_PTS > ((int64_t)1 << 31) - 1) return fail(BLDFM_ELIMIT, "too many points");
+  return 0;
+}
+
+int bldfm_sur_forward(const void* net, const float* surmet, int32_t nmet, const int32_t* blk_met,
+                      const double* kxy, int64_t npts, float* phi, hipStream_t stream) {
+  if (!net || !surmet || nmet <= 0 || !blk_met || !kxy || !phi) return fail(BLDFM_EINVAL, "bad sur_forward arguments");
+  int rc = sur_points_ok(npts);
+  if (rc) return rc;
+  rc = bldfm_launch_sur_forward((const uint8_t*)net, (const SurMet*)surmet, nmet, blk_met, (const double2*)kxy,
+                                nullptr, npts, nullptr, nullptr, nullptr, (float2*)phi, nullptr, 0, stream);
+  return rc ? launch_fail(rc, "surrogate forward") : 0;
+}
+
+int bldfm_sur_train_grad(const void* net, const float* surmet, int32_t nmet, const int32_t* blk_met,
+                         const double* kxy, const double* g, int64_t npts, void* ws, float* grad,
+                         float* loss, float* phi, hipStream_t stream) {
+  if (!net || !surmet || nmet <= 0 || !blk_met || !kxy || !g || !ws || !grad || !loss)
+    return fail(BLDFM_EINVAL, "bad sur_train_grad arguments");
+  int rc = sur_points_ok(npts);
+  if (rc) return rc;
+  __bf16* xs = (__bf16*)ws;
+  __bf16* zs = xs + (int64_t)(SUR_IN + SUR_NL * SUR_W) * npts;
+  __bf16* dzs = zs + (int64_t)SUR_NL * SUR_W * npts;
+  float2* dphi = (float2*)(dzs + (int64_t)SUR_NL * SUR_W * npts);
+  rc = bldfm_launch_sur_forward((const uint8_t*)net, (const SurMet*)surmet, nmet, blk_met, (const double2*)kxy,
+                                (const double2*)g, npts, xs, zs, dphi, (float2*)phi, loss, 1, stream);
+  if (rc) return launch_fail(rc, "surrogate forward");
+  const int chunk = npts % 2048 == 0 ? 2048 : SUR_PTS;
+  rc = bldfm_launch_sur_backward((const uint8_t*)net, npts, xs, zs, dphi, dzs, grad, chunk, stream);
+  return rc ? launch_fail(rc, "surrogate backward") : 0;
+}
+
+int bldfm_sur_adam(float* params, float* grad, float* m, float* v, void* net, float lr, float beta1,
+                   float beta2, float eps, int32_t step, hipStream_t stream) {
+  if (!params || !grad || !m || !v || !net || step < 1) return fail(BLDFM_EINVAL, "bad sur_adam arguments");
+  const float c1 = (float)(1.0 / (1.0 - pow((double)beta1, step)));
+  const float c2 = (float)(1.0 / (1.0 - pow((double)beta2, step)));
+  const int rc = bldfm_launch_sur_adam(params, grad, m, v, (uint8_t*)net, 1, lr, beta1, beta2, eps, c1, c2, stream);
+  return rc ? launch_fail(rc, "surrogate adam") : 0;
+}
+
+// Whether instance i of a plan lies inside a network's training hull.
+static bool sur_in_hull(const bldfm_plan* P, int i, const bldfm_sur_hull& H) {
+  const bldfm_instance& I = P->inst[i];
+  const InstDesc& D = P->desc[i];
+  if (!I.surrogate || !I.prof_from_met || !D.footprint || D.analytic) return false;
+  if (D.nslot != 1 || D.nfilled != 1 || D.max_level != I.met.n) return false;
+  const bldfm_met& m = I.met;
+  if (m.closure != H.closure || m.n != H.n || m.has_stretch || !m.has_domain_height || m.prsc != 1.0) return false;
+  if (!(fabs(m.domain_height / m.meas_height - H.top_ratio) <= 1e-9 * H.top_ratio)) return false;
+  const ProfSetup& s = P->setups[i];
+  const double il = 1.0 / s.mol;
+  if (!(s.ustar >= H.ustar_lo && s.ustar <= H.ustar_hi)) return false;
+  if (!(il >= H.invl_lo && il <= H.invl_hi)) return false;
+  if (!(s.z0 >= H.z0_lo && s.z0 <= H.z0_hi && s.z0 <= H.z0_zm_max * s.zm)) return false;
+  if (!(s.zm >= H.zm_lo && s.zm <= H.zm_hi)) return false;
+  const double kx = D.sx * (D.nlx / 2) * D.vx, ky = D.sy * (D.nly / 2) * D.vy;
+  return kx <= H.kmax && ky <= H.kmax;
+}
+
+int bldfm_plan_set_surrogate(bldfm_plan* P, const void* net, const bldfm_sur_hull* hull, double tol) {
+  if (!P) return fail(BLDFM_EINVAL, "null plan");
+  if (!net) {
+    P->dev.sur_net = nullptr;
+    return 0;
+  }
+  if (!hull || !(tol > 0.0)) return fail(BLDFM_EINVAL, "bad surrogate hull or tolerance");
+  if (!P->sur_any) return fail(BLDFM_EINVAL, "the plan has no surrogate-flagged footprint instance");
+  DeviceGuard guard(P->device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  std::vector<int32_t> ok(P->ninst);
+  for (int i = 0; i < P->ninst; ++i) ok[i] = sur_in_hull(P, i, *hull) ? 1 : 0;
+  hipError_t e = hipSuccess;
+  if (!P->sur_ok_dev) e = hipMalloc(&P->sur_ok_dev, sizeof(int32_t) * P->ninst);
+  if (e != hipSuccess) { P->sur_ok_dev = nullptr; return hip_fail(e, "hipMalloc(surrogate flags)"); }
+  e = hipMemcpy(P->sur_ok_dev, ok.data(), sizeof(int32_t) * P->ninst, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy(surrogate flags)");
+  P->dev.sur_ok = (const int32_t*)P->sur_ok_dev;
+  P->dev.sur_net = (const uint8_t*)net;
+  P->dev.sur_tol = tol;
+  return 0;
+}
+
+int bldfm_plan_sur_stats(const bldfm_plan* P, const void* workspace, double* err, int32_t* mode,
+                         hipStream_t stream) {
+  if (!P || !workspace) return fail(BLDFM_EINVAL, "bad surrogate stats arguments");
+  if (!P->sur_any || !P->dev.sur_net) return fail(BLDFM_EINVAL, "the plan serves no instance by its surrogate");
+  DeviceGuard guard(P->device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  const double2* ws = (const double2*)((const char*)workspace + align_up(P->tab_doubles * 8, 256));
+  std::vector<double2> gate(P->ninst);
+  std::vector<int32_t> md(P->ninst);
+  hipError_t e = hipMemcpyAsync(gate.data(), ws + P->off_gate, sizeof(double2) * P->ninst, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(md.data(), ws + P->off_mode, sizeof(int32_t) * P->ninst, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_fail(e, "reading surrogate gate state");
+  std::vector<int32_t> ok(P->ninst);
+  e = hipMemcpy(ok.data(), P->sur_ok_dev, sizeof(int32_t) * P->ninst, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "reading surrogate flags");
+  for (int i = 0; i < P->ninst; ++i) {
+    if (err) err[i] = (ok[i] && gate[i].y > 0.0) ? sqrt(gate[i].x / gate[i].y) : NAN;
+    if (mode) mode[i] = md[i];
+  }
   return 0;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include "bldfm_types.h"
 #include "bldfm_math.h"
+#include "bldfm_sur.h"
 
 // Device-side metadata of one plan (all pointers are device pointers owned
 // by the plan).
@@ -28,6 +29,15 @@ struct PlanDev {
   int32_t fft_threads;     // threads per FFT workgroup
   int32_t fft_lds_bytes;   // dynamic LDS per FFT workgroup
   int64_t tot_tab, tot_fwd_rows, tot_fwd_cols, tot_shoot, tot_nyq, tot_inv_cols, tot_inv_rows;
+  // learned surrogate of footprint instances (unused while sur_net is null)
+  const TileRec* t_sur;    // SUR_PTS-lane tiles of surrogate-flagged instances
+  const TileRec* t_spot;   // 256-sample tiles of the spot-check error gate
+  const SurMet* sur_met;   // ninst network inputs
+  const int32_t* sur_ok;   // ninst: 1 = flagged and inside the network's training hull
+  const uint8_t* sur_net;  // network buffer (SUR_NET_BYTES)
+  double sur_tol;          // largest accepted spot-check relative L2 error
+  int32_t sur_stride;      // one spot-checked lane in sur_stride
+  int64_t tot_sur, tot_spot;
 };
 
 struct ExecBufs {
@@ -37,12 +47,29 @@ struct ExecBufs {
   void* flx;
   double* tab;         // level tables (workspace)
   double2* ws;         // complex workspace
+  double2* gate;       // ninst spot-check sums (|surrogate - solver|^2, |solver|^2)
+  int32_t* mode;       // ninst: 0 surrogate accepted, 1 FP64 solver; null: every instance solved
 };
 
 // ev (optional): 7 events recorded at the phase boundaries
 // [level table | forward transform | shooting | Nyquist fix-up | y-pass | x-pass]
 int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream,
                      hipEvent_t* ev = nullptr);
+// Surrogate inference over the lanes of accepted instances (packed spectrum).
+int bldfm_launch_sur_plan(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream);
+// Surrogate training and evaluation on point lists (N a multiple of SUR_PTS).
+int bldfm_launch_sur_forward(const uint8_t* net, const SurMet* mets, int nmet, const int32_t* blk_met,
+                             const double2* kxy, const double2* gt, int64_t N, __bf16* xs, __bf16* zs,
+                             float2* dphi, float2* phi, float* loss, int train, hipStream_t stream);
+int bldfm_launch_sur_backward(const uint8_t* net, int64_t N, const __bf16* xs, const __bf16* zs,
+                              const float2* dphi, __bf16* dzs, float* grad, int chunk, hipStream_t stream);
+int bldfm_launch_sur_adam(float* p, float* g, float* m, float* v, uint8_t* net, int update, float lr,
+                          float b1, float b2, float eps, float c1, float c2, hipStream_t stream);
+// FP64 point solver: level tables from met setups, then G at the measurement level.
+int bldfm_launch_tables(const ProfSetup* S, int nmet, int rows, double* tab, hipStream_t stream);
+int bldfm_launch_transfer(const double* tab, int rows, const int32_t* nzs, const int32_t* lvls, int nmet,
+                          const int32_t* blk_met, int nblk, const double2* kxy, double2* g, int exact,
+                          hipStream_t stream);
 int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream);
 // F0/F1: device-resident plans of the two axis lengths n0 (columns) and n1 (rows)
 int bldfm_launch_fft2(const FftPlan* F0, int32_t n0, const double2* tw0, const FftPlan* F1, int32_t n1,

@@ -16,6 +16,7 @@
 // sums; every kernel is a grid-stride loop over its tiles.
 #include "bldfm_internal.h"
 #include "bldfm_fft.h"
+#include "bldfm_march.h"
 
 #define SHOOT_THREADS 256
 #define FFT_SMALL_E 8     // values per thread per stage, 256-thread FFT kernels (n <= 2048)
@@ -26,20 +27,6 @@
 __device__ __forceinline__ int64_t xcd_swizzle(int64_t b, int64_t n) {
   const int64_t q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
   return x * q + (x < r ? x : r) + i;
-}
-
-// Uniform per-tile data read through the constant address space (scalar
-// loads into SGPRs): tile records, instance descriptors, FFT plans.
-typedef const __attribute__((address_space(4))) InstDesc cdesc_t;
-typedef const __attribute__((address_space(4))) FftPlan cplan_t;
-typedef const __attribute__((address_space(4))) TileRec ctile_t;
-
-__device__ __forceinline__ TileRec tile_rec(const TileRec* recs, int64_t t) {
-  const ctile_t* r = (const ctile_t*)(recs + t);
-  TileRec o;
-  o.inst = r->inst;
-  o.local = r->local;
-  return o;
 }
 
 // ---------------------------------------------------------------- level table
@@ -55,6 +42,46 @@ __device__ __forceinline__ TileRec tile_rec(const TileRec* recs, int64_t t) {
 // b = b0 - sb T h3, c = T h - sc T^2 h3.
 // Entry nz-1 (top, src/bldfm/solver.py:164-174):
 //   {Kx/Kz, Ky/Kz, u/Kz, v/Kz, Kz, 1/Kz} (products with 1/Kz as the reference forms them)
+// One level-table row from the profile values (z,u,v,Kx,Ky,Kz) of level i
+// (cur) and level i+1 (nxt, unused for the top row).
+__device__ __forceinline__ void fill_level_row(const double* cur, const double* nxt, bool top,
+                                               double* __restrict__ T) {
+#pragma clang fp contract(off)
+  const double kz = cur[5];
+  const double kinv = 1.0 / kz;
+  if (!top) {
+    const double h = nxt[0] - cur[0];
+    const double h2 = h * h;
+    const double h3 = h * h2;
+    const double sixth = 1.0 / 6.0;
+    const double sa = 0.5 * kinv, sb = sixth * (kinv * kinv), sc = sixth * kinv;
+    T[0] = cur[3];
+    T[1] = cur[4];
+    T[2] = cur[1];
+    T[3] = cur[2];
+    T[4] = sa;
+    T[5] = h2;
+    T[6] = -kinv * h;
+    T[7] = sb;
+    T[8] = h3;
+    T[9] = h;
+    T[10] = sc;
+    T[11] = 0.5 / kz + 0.5 / nxt[5];
+    T[12] = sa * h2;
+    T[13] = sb * h3;
+    T[14] = sc * h3;
+    T[15] = 0.0;
+  } else {
+    T[0] = cur[3] * kinv;
+    T[1] = cur[4] * kinv;
+    T[2] = cur[1] * kinv;
+    T[3] = cur[2] * kinv;
+    T[4] = kz;
+    T[5] = kinv;
+    for (int c = 6; c < BLDFM_LEVEL_STRIDE; ++c) T[c] = 0.0;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_level_table(PlanDev P, ExecBufs B) {
 #pragma clang fp contract(off)
   for (int64_t t = blockIdx.x; t < P.tot_tab; t += gridDim.x) {
@@ -76,184 +103,8 @@ __global__ __launch_bounds__(256) void k_level_table(PlanDev P, ExecBufs B) {
       if (i + 1 < nz)
         for (int c = 0; c < 6; ++c) nxt[c] = prof[(int64_t)c * nz + i + 1];
     }
-    double* T = B.tab + D.off_tab + (int64_t)i * BLDFM_LEVEL_STRIDE;
-    const double kz = cur[5];
-    const double kinv = 1.0 / kz;
-    if (i < nz - 1) {
-      const double h = nxt[0] - cur[0];
-      const double h2 = h * h;
-      const double h3 = h * h2;
-      const double sixth = 1.0 / 6.0;
-      const double sa = 0.5 * kinv, sb = sixth * (kinv * kinv), sc = sixth * kinv;
-      T[0] = cur[3];
-      T[1] = cur[4];
-      T[2] = cur[1];
-      T[3] = cur[2];
-      T[4] = sa;
-      T[5] = h2;
-      T[6] = -kinv * h;
-      T[7] = sb;
-      T[8] = h3;
-      T[9] = h;
-      T[10] = sc;
-      T[11] = 0.5 / kz + 0.5 / nxt[5];
-      T[12] = sa * h2;
-      T[13] = sb * h3;
-      T[14] = sc * h3;
-      T[15] = 0.0;
-    } else {
-      T[0] = cur[3] * kinv;
-      T[1] = cur[4] * kinv;
-      T[2] = cur[1] * kinv;
-      T[3] = cur[2] * kinv;
-      T[4] = kz;
-      T[5] = kinv;
-      for (int c = 6; c < BLDFM_LEVEL_STRIDE; ++c) T[c] = 0.0;
-    }
+    fill_level_row(cur, nxt, i == nz - 1, B.tab + D.off_tab + (int64_t)i * BLDFM_LEVEL_STRIDE);
   }
-}
-
-// ---------------------------------------------------------------- shooting
-struct Lane {
-  int kind, fx, fy, nidx;
-  double lx, ly;
-};
-
-__device__ __forceinline__ bool lane_setup(const cdesc_t& D, int lane, Lane& L) {
-  if (lane >= D.lanes) return false;
-  L.fx = 0; L.fy = 0; L.nidx = -1;
-  L.kind = decode_lane(lane, D.nlx, D.nly, &L.fx, &L.fy, &L.nidx);
-  if (L.kind == LANE_IDLE) return false;
-  // lx = ((2 pi)/dx)/nxe * (f * 1/(nl*(1/nl)))   (numpy fftfreq, solver.py:148-153)
-  L.lx = xmul(D.sx, xmul((double)L.fx, D.vx));
-  L.ly = xmul(D.sy, xmul((double)L.fy, D.vy));
-  return true;
-}
-
-// Packed output of one mode (after complex64 rounding and phase shift).
-__device__ __forceinline__ void emit_mode(const cdesc_t& D, const Lane& L, double2* __restrict__ ws,
-                                          int s, double2 Xp, double2 Xq, double2 sh) {
-  if (D.single) { Xp = round_c64(Xp); Xq = round_c64(Xq); }
-  if (D.has_shift) { Xp = ncmul(Xp, sh); Xq = ncmul(Xq, sh); }
-  if (L.kind == LANE_HALF) {
-    const int cx = compact_of_freq(L.fx, D.nxb), cy = compact_of_freq(L.fy, D.nyb);
-    const int mx = compact_mirror(cx, D.nxb), my = compact_mirror(cy, D.nyb);
-    double2* Z = ws + D.off_z + (int64_t)s * D.nxb * D.nyb;
-    // Z(k) = Xp + i Xq ; Z(-k) = conj(Xp) + i conj(Xq)
-    Z[(int64_t)cx * D.nyb + cy] = cmk(Xp.x - Xq.y, Xp.y + Xq.x);
-    Z[(int64_t)mx * D.nyb + my] = cmk(Xp.x + Xq.y, Xq.x - Xp.y);
-  } else {  // LANE_NYQ: keep X for the Hermitian fix-up
-    double2* N = ws + D.off_nyq + ((int64_t)s * (D.nlx + D.nly) + L.nidx) * 2;
-    N[0] = Xp;
-    N[1] = Xq;
-  }
-}
-
-__device__ __forceinline__ void emit_origin(const cdesc_t& D, double2* __restrict__ ws, int s,
-                                            double p00, double q00) {
-  if (D.single) { p00 = (double)(float)p00; q00 = (double)(float)q00; }
-  ws[D.off_z + (int64_t)s * D.nxb * D.nyb] = cmk(p00, q00);
-}
-
-// e^{i(lx*shx + ly*shy)} (src/bldfm/solver.py:254-262)
-__device__ __forceinline__ double2 phase_shift(const cdesc_t& D, const Lane& L) {
-  double s = 0.0, c = 1.0;
-  if (D.has_shift) {
-    const double th = xmul(L.lx, D.shx) + xmul(L.ly, D.shy);
-    sincos(th, &s, &c);
-  }
-  return cmk(c, s);
-}
-
-__device__ __forceinline__ double2 ivp2_init(const cdesc_t& D, const Lane& L,
-                                             const double2* __restrict__ ws) {
-  if (D.footprint) return cmk(D.q00, 0.0);
-  const int jx = trunc_of_freq(L.fx, D.nlx), jy = trunc_of_freq(L.fy, D.nly);
-  return ws[D.off_q0 + (int64_t)jx * D.nly + jy];
-}
-
-// One step of both initial-value problems,
-// [p; q] <- [[a, b], [c, a]] [p; q]  (src/bldfm/solver.py:357-368).
-// EXACT: every product and sum rounded separately, in the reference's order
-//   T = (-(Kx*lx2 + Ky*ly2), -(u*lx + v*ly))
-//   a = 1 - (sa*T)*h2,  b = b0 - (sb*T)*h3,  c = T*h - (sc*T^2)*h3
-// otherwise: the same algebra with fused multiply-adds.
-// The level table and slot map are uniform across a workgroup and read-only
-// during the march: reading them through the constant address space lets
-// the compiler fetch them with scalar loads into SGPRs.
-typedef const __attribute__((address_space(4))) double ctab_t;
-typedef const __attribute__((address_space(4))) int32_t cint_t;
-
-// One level-table row in registers (SGPRs: the row is uniform).
-struct Row {
-  double kx, ky, u, v, sa, h2, b0, sb, h3, h, sc, A, B1, C;
-};
-
-__device__ __forceinline__ Row load_row(ctab_t* __restrict__ T) {
-  Row r;
-  r.kx = T[0]; r.ky = T[1]; r.u = T[2]; r.v = T[3];
-  r.sa = T[4]; r.h2 = T[5]; r.b0 = T[6]; r.sb = T[7]; r.h3 = T[8]; r.h = T[9]; r.sc = T[10];
-  r.A = T[12]; r.B1 = T[13]; r.C = T[14];
-  return r;
-}
-
-template <bool EXACT>
-__device__ __forceinline__ void march_step(const Row& R, double lx, double ly, double lx2, double ly2,
-                                           double2& p1, double2& q1, double2& p2, double2& q2);
-
-template <>
-__device__ __forceinline__ void march_step<true>(const Row& R, double lx, double ly, double lx2,
-                                                 double ly2, double2& p1, double2& q1, double2& p2,
-                                                 double2& q2) {
-#pragma clang fp contract(off)
-  const double tr = -(R.kx * lx2 + R.ky * ly2);
-  const double ti = -(R.u * lx + R.v * ly);
-  const double ar = 1.0 - (R.sa * tr) * R.h2, ai = -((R.sa * ti) * R.h2);
-  const double br = R.b0 - (R.sb * tr) * R.h3, bi = -((R.sb * ti) * R.h3);
-  const double t2r = tr * tr - ti * ti;
-  const double tri = tr * ti;
-  const double t2i = tri + tri;
-  const double cr = tr * R.h - (R.sc * t2r) * R.h3, ci = ti * R.h - (R.sc * t2i) * R.h3;
-  const double2 a = cmk(ar, ai), b = cmk(br, bi), c = cmk(cr, ci);
-  const double2 np1 = xcmuladd(a, p1, b, q1);
-  const double2 nq1 = xcmuladd(c, p1, a, q1);
-  const double2 np2 = xcmuladd(a, p2, b, q2);
-  const double2 nq2 = xcmuladd(c, p2, a, q2);
-  p1 = np1; q1 = nq1; p2 = np2; q2 = nq2;
-}
-
-template <>
-__device__ __forceinline__ void march_step<false>(const Row& R, double lx, double ly, double lx2,
-                                                  double ly2, double2& p1, double2& q1, double2& p2,
-                                                  double2& q2) {
-  const double tr = -(R.kx * lx2 + R.ky * ly2);
-  const double ti = -(R.u * lx + R.v * ly);
-  const double2 a = cmk(1.0 - R.A * tr, -(R.A * ti));
-  const double2 b = cmk(R.b0 - R.B1 * tr, -(R.B1 * ti));
-  const double t2r = tr * tr - ti * ti, t2i = 2.0 * tr * ti;
-  const double2 c = cmk(R.h * tr - R.C * t2r, R.h * ti - R.C * t2i);
-  const double2 np1 = cadd(cmul(a, p1), cmul(b, q1));
-  const double2 nq1 = cadd(cmul(c, p1), cmul(a, q1));
-  const double2 np2 = cadd(cmul(a, p2), cmul(b, q2));
-  const double2 nq2 = cadd(cmul(c, p2), cmul(a, q2));
-  p1 = np1; q1 = nq1; p2 = np2; q2 = nq2;
-}
-
-// Radiation condition q = Kz lambda p at the top and the shooting coefficient
-// alpha = -(q2 - Kz lambda p2)/(q1 - Kz lambda p1)
-// (src/bldfm/solver.py:164-174, 228-230), rounded as numpy rounds it.
-__device__ __forceinline__ double2 top_lambda(const double* __restrict__ tt, double lx, double ly,
-                                              double lx2, double ly2) {
-#pragma clang fp contract(off)
-  return csqrt_exact(cmk(tt[0] * lx2 + tt[1] * ly2, tt[2] * lx + tt[3] * ly));
-}
-
-__device__ __forceinline__ double2 shoot_alpha(const double* __restrict__ tt, double2 lam, double2 p1,
-                                               double2 q1, double2 p2, double2 q2) {
-  const double2 kl = xcscale(tt[4], lam);
-  const double2 num = xcsub(q2, ncmul(kl, p2));
-  const double2 den = xcsub(q1, ncmul(kl, p1));
-  return cdiv(cmk(-num.x, -num.y), den);
 }
 
 // Degenerate mode (0,0): flux conserved, p by the trapezoid rule in z
@@ -283,6 +134,7 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
     const TileRec tr = tile_rec(P.t_shoot, t);
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     if (D.variant != variant || D.exact != (int)EXACT) continue;
+    if (B.mode && B.mode[tr.inst] == 0) continue;  // accepted surrogate instance
     Lane L;
     const int lane = tr.local * SHOOT_THREADS + (int)threadIdx.x;
     if (!lane_setup(D, lane, L)) continue;
@@ -358,6 +210,7 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot_analytic(PlanDev P, Exe
     const TileRec tr = tile_rec(P.t_shoot, t);
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     if (D.variant != variant) continue;
+    if (B.mode && B.mode[tr.inst] == 0) continue;
     Lane L;
     const int lane = tr.local * SHOOT_THREADS + (int)threadIdx.x;
     if (!lane_setup(D, lane, L)) continue;
@@ -411,6 +264,137 @@ __global__ __launch_bounds__(256) void k_nyq_fix(PlanDev P, ExecBufs B) {
     if (n2 >= 0) { A = cadd(A, cconj(N[2 * n2])); Q = cadd(Q, cconj(N[2 * n2 + 1])); }
     B.ws[D.off_z + ((int64_t)s * D.nxb + cx) * D.nyb + cy] = cmk(0.5 * (A.x - Q.y), 0.5 * (A.y + Q.x));
   }
+}
+
+// ---------------------------------------------------------------- point solver
+// Level tables of nmet met samples for the point solver, `rows` rows per
+// sample (rows >= the sample's level count).
+__global__ __launch_bounds__(256) void k_tables_met(const ProfSetup* __restrict__ S, int nmet, int rows,
+                                                    double* __restrict__ tab) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= (int64_t)nmet * rows) return;
+  const int m = (int)(g / rows), i = (int)(g % rows);
+  const ProfSetup& s = S[m];
+  const int nz = s.nlev;
+  if (i >= nz) return;
+  double cur[6], nxt[6];
+  prof_point(s, i, cur);
+  if (i + 1 < nz) prof_point(s, i + 1, nxt);
+  fill_level_row(cur, nxt, i == nz - 1, tab + g * BLDFM_LEVEL_STRIDE);
+}
+
+// Flux transfer G = q^(z_lvl) / q^(0) of single modes: the shooting solve
+// of the footprint mode (both IVPs, radiation condition, shooting
+// coefficient) for a unit surface flux.  Blocks of 256 points share one met
+// sample (its table rows are uniform, read through scalar loads).  Invalid
+// block metadata yields NaN instead of an out-of-range read.
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_transfer(const double* __restrict__ tab, int rows,
+                                                  const int32_t* __restrict__ nzs,
+                                                  const int32_t* __restrict__ lvls, int nmet,
+                                                  const int32_t* __restrict__ blk_met, int nblk,
+                                                  const double2* __restrict__ kxy, double2* __restrict__ g) {
+  for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const int64_t p = (int64_t)b * 256 + threadIdx.x;
+    const int m = blk_met[b];
+    const int nz = (m >= 0 && m < nmet) ? nzs[m] : 0;
+    const int lvl = (m >= 0 && m < nmet) ? lvls[m] : -1;
+    if (nz < 2 || nz > rows || lvl < 0 || lvl >= nz) {
+      g[p] = cmk(__builtin_nan(""), __builtin_nan(""));
+      continue;
+    }
+    const double* T = tab + (int64_t)m * rows * BLDFM_LEVEL_STRIDE;
+    ctab_t* ct = (ctab_t*)T;
+    const double2 k = kxy[p];
+    const double lx = k.x, ly = k.y, lx2 = xmul(lx, lx), ly2 = xmul(ly, ly);
+    double2 p1 = cmk(1.0, 0.0), q1 = cmk(0.0, 0.0), p2 = cmk(0.0, 0.0), q2 = cmk(1.0, 0.0);
+    double2 sq1 = q1, sq2 = q2;
+    Row nxt = load_row(ct);
+    for (int i = 0; i < nz - 1; ++i) {
+      if (i == lvl) { sq1 = q1; sq2 = q2; }
+      const Row cur = nxt;
+      if (i + 1 < nz - 1) nxt = load_row(ct + (int64_t)(i + 1) * BLDFM_LEVEL_STRIDE);
+      march_step<EXACT>(cur, lx, ly, lx2, ly2, p1, q1, p2, q2);
+    }
+    if (lvl == nz - 1) { sq1 = q1; sq2 = q2; }
+    const double* tt = T + (int64_t)(nz - 1) * BLDFM_LEVEL_STRIDE;
+    const double2 alpha = shoot_alpha(tt, top_lambda(tt, lx, ly, lx2, ly2), p1, q1, p2, q2);
+    g[p] = (lx == 0.0 && ly == 0.0) ? cmk(1.0, 0.0) : xcadd(ncmul(alpha, sq1), sq2);
+  }
+}
+
+// ---------------------------------------------------------------- surrogate gate
+// Per-instance gate state before the surrogate runs: sums cleared, accepted
+// (mode 0) if flagged and inside the training hull, else solved (mode 1).
+__global__ __launch_bounds__(256) void k_sur_clear(PlanDev P, ExecBufs B) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.ninst) return;
+  B.gate[i] = cmk(0.0, 0.0);
+  B.mode[i] = P.sur_ok[i] ? 0 : 1;
+}
+
+// Spot check: every sur_stride-th lane of an accepted instance is solved in
+// FP64 and compared with the surrogate's packed output X_q = -i Z(k).
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_sur_spot(PlanDev P, ExecBufs B) {
+  __shared__ double sd[4], ss[4];
+  for (int64_t t = blockIdx.x; t < P.tot_spot; t += gridDim.x) {
+    const TileRec tr = tile_rec(P.t_spot, t);
+    const int inst = tr.inst;
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + inst);
+    if (B.mode[inst] != 0 || D.exact != (int)EXACT) continue;  // uniform over the workgroup
+    const int off = (int)((inst * 61LL) % P.sur_stride);
+    const int lane = off + (tr.local * 256 + (int)threadIdx.x) * P.sur_stride;
+    double d2 = 0.0, s2 = 0.0;
+    Lane L;
+    if (lane_setup(D, lane, L) && L.kind == LANE_HALF) {
+      const double* __restrict__ tab = B.tab + D.off_tab;
+      ctab_t* __restrict__ ctab = (ctab_t*)tab;
+      const int nz = D.nz, lvl = D.max_level;
+      const double lx = L.lx, ly = L.ly, lx2 = xmul(lx, lx), ly2 = xmul(ly, ly);
+      double2 p1 = cmk(1.0, 0.0), q1 = cmk(0.0, 0.0), p2 = cmk(0.0, 0.0), q2 = cmk(D.q00, 0.0);
+      double2 sq1 = q1, sq2 = q2;
+      Row nxt = load_row(ctab);
+      for (int i = 0; i < nz - 1; ++i) {
+        if (i == lvl) { sq1 = q1; sq2 = q2; }
+        const Row cur = nxt;
+        if (i + 1 < nz - 1) nxt = load_row(ctab + (int64_t)(i + 1) * BLDFM_LEVEL_STRIDE);
+        march_step<EXACT>(cur, lx, ly, lx2, ly2, p1, q1, p2, q2);
+      }
+      if (lvl == nz - 1) { sq1 = q1; sq2 = q2; }
+      const double* tt = tab + (int64_t)(nz - 1) * BLDFM_LEVEL_STRIDE;
+      const double2 alpha = shoot_alpha(tt, top_lambda(tt, lx, ly, lx2, ly2), p1, q1, p2, q2);
+      double2 xq = xcadd(ncmul(alpha, sq1), sq2);
+      if (D.has_shift) xq = ncmul(xq, phase_shift(D, L));
+      const int cx = compact_of_freq(L.fx, D.nxb), cy = compact_of_freq(L.fy, D.nyb);
+      const double2 z = B.ws[D.off_z + (int64_t)cx * D.nyb + cy];
+      const double er = z.y - xq.x, ei = -z.x - xq.y;
+      d2 = er * er + ei * ei;
+      s2 = xq.x * xq.x + xq.y * xq.y;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      d2 += __shfl_xor(d2, o);
+      s2 += __shfl_xor(s2, o);
+    }
+    if ((threadIdx.x & 63) == 0) { sd[threadIdx.x >> 6] = d2; ss[threadIdx.x >> 6] = s2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      atomicAdd(&B.gate[inst].x, sd[0] + sd[1] + sd[2] + sd[3]);
+      atomicAdd(&B.gate[inst].y, ss[0] + ss[1] + ss[2] + ss[3]);
+    }
+    __syncthreads();
+  }
+}
+
+// Gate decision: an accepted instance whose spot-check relative L2 error
+// exceeds sur_tol (or is not finite) falls back to the FP64 solver.
+__global__ __launch_bounds__(256) void k_sur_gate(PlanDev P, ExecBufs B) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.ninst || B.mode[i] != 0) return;
+  const double2 s = B.gate[i];
+  const double err = s.y > 0.0 ? sqrt(s.x / s.y) : 0.0;
+  if (!(err <= P.sur_tol)) B.mode[i] = 1;
 }
 
 // ---------------------------------------------------------------- FFT passes
@@ -521,12 +505,14 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
     band_load(smem, B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, D.nxb, F.n, tid, T);
     bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, !D.footprint, tid, T);
     const int64_t o = D.off_out + local * D.nx;
+    // instances served by the surrogate carry no concentration spectrum
+    const double cm = (B.mode && B.mode[tr.inst] == 0) ? 0.0 : 1.0;
     if (D.out_f32) {
       float* conc = (float*)B.conc + o;
       float* flx = (float*)B.flx + o;
       for (int x = tid; x < D.nx; x += T) {
         const double2 w = smem[D.px + x];
-        conc[x] = (float)w.x;
+        conc[x] = (float)(cm * w.x);
         flx[x] = (float)w.y;
       }
     } else {
@@ -534,7 +520,7 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
       double* flx = (double*)B.flx + o;
       for (int x = tid; x < D.nx; x += T) {
         const double2 w = smem[D.px + x];
-        conc[x] = w.x;
+        conc[x] = cm * w.x;
         flx[x] = w.y;
       }
     }
@@ -576,6 +562,22 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
     FFT_LAUNCH(k_fwd_cols, P.tot_fwd_cols)
   }
   MARK(2)
+  const bool sur = P.sur_net != nullptr && B.mode != nullptr && P.tot_sur > 0;
+  if (sur) {
+    const unsigned gi = (unsigned)((P.ninst + 255) / 256);
+    k_sur_clear<<<gi, 256, 0, stream>>>(P, B);
+    LAUNCH_CHECK();
+    const int rc = bldfm_launch_sur_plan(P, B, grid_cap, stream);
+    if (rc) return rc;
+    if (P.tot_spot > 0) {
+      k_sur_spot<true><<<grid_for(P.tot_spot, grid_cap), 256, 0, stream>>>(P, B);
+      LAUNCH_CHECK();
+      k_sur_spot<false><<<grid_for(P.tot_spot, grid_cap), 256, 0, stream>>>(P, B);
+      LAUNCH_CHECK();
+    }
+    k_sur_gate<<<gi, 256, 0, stream>>>(P, B);
+    LAUNCH_CHECK();
+  }
   const unsigned gs = grid_for(P.tot_shoot, grid_cap);
 #define SHOOT(V, MAXS, RECOMP, EX)                                                    \
   if (P.variant_used[V][EX]) {                                                        \
@@ -761,6 +763,25 @@ __global__ __launch_bounds__(256) void k_math_probe(int op, const double* __rest
 int bldfm_launch_math_probe(int op, const double* a, const double* b, int64_t n, double* out, hipStream_t stream) {
   if (n <= 0) return 0;
   k_math_probe<<<grid_for((n + 255) / 256, 4096), 256, 0, stream>>>(op, a, b, n, out);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int bldfm_launch_tables(const ProfSetup* S, int nmet, int rows, double* tab, hipStream_t stream) {
+  const int64_t n = (int64_t)nmet * rows;
+  k_tables_met<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(S, nmet, rows, tab);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int bldfm_launch_transfer(const double* tab, int rows, const int32_t* nzs, const int32_t* lvls, int nmet,
+                          const int32_t* blk_met, int nblk, const double2* kxy, double2* g, int exact,
+                          hipStream_t stream) {
+  const unsigned grid = (unsigned)(nblk < (1 << 20) ? nblk : (1 << 20));
+  if (exact)
+    k_transfer<true><<<grid, 256, 0, stream>>>(tab, rows, nzs, lvls, nmet, blk_met, nblk, kxy, g);
+  else
+    k_transfer<false><<<grid, 256, 0, stream>>>(tab, rows, nzs, lvls, nmet, blk_met, nblk, kxy, g);
   LAUNCH_CHECK();
   return 0;
 }

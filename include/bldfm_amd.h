@@ -77,7 +77,8 @@ typedef struct bldfm_instance {
   int32_t fast;              /* 0: reproduce the reference's rounding in the vertical
                                 march (default); 1: fused multiply-adds (faster,
                                 agrees to the roundoff of the shooting cancellation) */
-  int32_t reserved;          /* must be 0 */
+  int32_t surrogate;         /* 1: may be served by the plan's surrogate network
+                                (bldfm_plan_set_surrogate); 0: always solved */
   double dx, dy;             /* xmax/nx, ymax/ny */
   double shift_x, shift_y;   /* xm+halo, ym+halo (footprint) or xm-xmax/2, ym-ymax/2 */
   double srf_bg_conc;        /* p000 */
@@ -189,6 +190,91 @@ int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* 
  */
 int bldfm_math_probe(int32_t op, const double* a, const double* b, int64_t n, double* out,
                      hipStream_t stream);
+
+
+/* ------------------------------------------------------------------------
+ * Learned spectral surrogate of the footprint transfer function.
+ * No counterpart in the reference.  A coordinate MLP maps one Fourier mode
+ * of one footprint solve (wavenumber and met scalars) to the flux transfer
+ * G(k) = q^(zm; k)/q^(0; k) that steady_state_transport_solver computes in
+ * footprint mode (src/bldfm/solver.py:132-134, 220-235, 307-374).  The FP64
+ * solver generates its training data, checks it on spot-check modes, and
+ * stays its fallback.
+ */
+#define BLDFM_SUR_SPOT_STRIDE 100   /* one spot-checked mode in 100 (1 %) */
+
+/* Box of met conditions and the vertical-grid convention a network was
+ * trained on.  Instances outside it are always solved. */
+typedef struct bldfm_sur_hull {
+  double ustar_lo, ustar_hi;   /* friction velocity [m/s] */
+  double invl_lo, invl_hi;     /* 1/L [1/m] */
+  double z0_lo, z0_hi;         /* roughness length [m] */
+  double z0_zm_max;            /* largest z0/zm */
+  double zm_lo, zm_hi;         /* measurement height [m] */
+  double kmax;                 /* largest |kx|, |ky| [rad/m] */
+  double top_ratio;            /* domain_height / meas_height */
+  int32_t n;                   /* grid points below zm */
+  int32_t closure;             /* BLDFM_CLOSURE_* */
+} bldfm_sur_hull;
+
+/* Sizes: fp32 parameter count, network buffer bytes, training workspace
+ * bytes per point, points per training block (npts must be a multiple). */
+int bldfm_sur_sizes(int64_t* nparam, int64_t* net_bytes, int64_t* train_bytes_per_point,
+                    int32_t* block_points);
+
+/* Network inputs of nmet met samples, 76 floats each: u*, 1/L, ln z0, zm,
+ * |U|, wind direction [rad], zm/L, ln(z0/zm) (with the derived z0 / u* of
+ * vertical_profiles, src/bldfm/pbl_model.py:65-77), the unit wind vector,
+ * zm, the number nw of vertical intervals below zm, then 32 pairs
+ * (h_j, U_j/K_j) of those intervals (the WKB prior of the network). */
+int bldfm_sur_met(const bldfm_met* met, int32_t nmet, float* surmet);
+
+/* Level tables of nmet met samples on the device (rows doubles*16 each,
+ * rows >= every sample's level count); nz (host, nmet) receives the counts. */
+int bldfm_transfer_tables(int device, const bldfm_met* met, int32_t nmet, int32_t rows, double* tab,
+                          int32_t* nz, hipStream_t stream);
+
+/* FP64 flux transfer of single modes: block b (256 points, kxy as (lx, ly)
+ * pairs) uses met sample blk_met[b] and its measurement level lvl[m].
+ * exact = 1 reproduces the reference's rounding of the march. */
+int bldfm_transfer_solve(const double* tab, int32_t rows, const int32_t* nz, const int32_t* lvl,
+                         int32_t nmet, const int32_t* blk_met, int32_t nblk, const double* kxy,
+                         double* g, int32_t exact, hipStream_t stream);
+
+/* Packs fp32 parameters (bldfm_sur_sizes' nparam) into a network buffer
+ * whose first 1024 bytes (features and normalisation) the caller wrote. */
+int bldfm_sur_pack(const float* params, void* net, hipStream_t stream);
+
+/* Phi = -log G (two floats per point) of npts points; block b of
+ * block_points points uses surmet[blk_met[b]]. */
+int bldfm_sur_forward(const void* net, const float* surmet, int32_t nmet, const int32_t* blk_met,
+                      const double* kxy, int64_t npts, float* phi, hipStream_t stream);
+
+/* Loss against solver transfers g and its gradient: loss (one float,
+ * accumulated) and grad (nparam floats: weights accumulated, biases and
+ * output layer overwritten).  ws: npts * train_bytes_per_point bytes. */
+int bldfm_sur_train_grad(const void* net, const float* surmet, int32_t nmet, const int32_t* blk_met,
+                         const double* kxy, const double* g, int64_t npts, void* ws, float* grad,
+                         float* loss, float* phi, hipStream_t stream);
+
+/* Adam step t (1-based) on fp32 master parameters with moments m, v; clears
+ * grad and repacks the network buffer. */
+int bldfm_sur_adam(float* params, float* grad, float* m, float* v, void* net, float lr, float beta1,
+                   float beta2, float eps, int32_t step, hipStream_t stream);
+
+/* Lets a plan serve its surrogate-flagged instances with the network in
+ * `net` (null: none): each execution evaluates the network on every mode of
+ * those inside `hull`, solves one mode in BLDFM_SUR_SPOT_STRIDE in FP64,
+ * and solves the whole instance instead when the spot-check relative L2
+ * error exceeds tol.  Served instances get the flux footprint; their conc
+ * output is 0. */
+int bldfm_plan_set_surrogate(bldfm_plan* plan, const void* net, const bldfm_sur_hull* hull, double tol);
+
+/* After an execution (synchronises `stream`): per instance the spot-check
+ * relative L2 error (NaN when not served) and the path taken
+ * (0 surrogate, 1 solver). */
+int bldfm_plan_sur_stats(const bldfm_plan* plan, const void* workspace, double* err, int32_t* mode,
+                         hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,210 @@
+"""Learned surrogate on the GPU: the FP64 point solver against the oracle's
+footprint spectra, the bf16-MFMA forward and hand-written backward against
+the numpy network with the same roundings, Adam against numpy, the error
+gate and its FP64 fallback inside a plan.
+
+Bars: point solver vs oracle transfer rel-L2 <= 1e-8 (FP64, unit instead of
+1/N surface flux); network output vs numpy (bf16 storage mirrored) rel-L2
+<= 1e-3, loss rel 1e-3, every gradient block rel-L2 <= 1e-2; Adam rel 1e-5;
+fallback footprints bitwise equal to the plain batch.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import solver as osolver
+from oracle import surrogate as osur
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def metset():
+    import torch
+
+    assert torch.cuda.is_available()
+    from bldfm_amd import surrogate as sur
+
+    return sur.MetSet(sur.sample_met(8, seed=7))
+
+
+@pytest.fixture(scope="module")
+def net(metset):
+    from bldfm_amd import surrogate as sur
+
+    mu, sd = sur.met_normalisation(metset)
+    return sur.SurrogateNet(mu, sd, seed=3)
+
+
+def test_point_solver_matches_oracle_spectra(metset):
+    """G from bldfm_transfer_solve equals the oracle footprint spectrum
+    q^(zm)/(q00 e^{i k.x}) on a whole mode lattice."""
+    import torch
+
+    from bldfm_amd import surrogate as sur
+    from oracle import pbl as opbl
+
+    s = metset.samples
+    for m in (0, 3):
+        d = np.deg2rad(s["dir"][m])
+        wind = (-s["speed"][m] * np.sin(d), -s["speed"][m] * np.cos(d))
+        zm = s["zm"][m]
+        z, prof = opbl.vertical_profiles(26, zm, wind, ustar=s["ustar"][m], mol=1.0 / s["invl"][m],
+                                         domain_height=zm * 100.0 / 15.0)
+        dom, modes = (500.0, 500.0), (64, 64)
+        _, (tp, tq) = osolver.steady_state_transport_solver(
+            np.zeros((64, 64)), z, prof, dom, 26, modes=modes, meas_pt=(250.0, 250.0), footprint=True,
+            precision="double", _return_spectra=True)
+        lat = sur.lattice(dom, modes)
+        nxe = 64 * 3
+        shift = np.exp(1j * (lat[:, 0] * 750.0 + lat[:, 1] * 750.0))
+        g_or = tq[0].ravel() / shift * nxe * nxe
+        P = lat.shape[0]
+        kxy = torch.from_numpy(lat).cuda()
+        blk = torch.full((P // 256,), m, dtype=torch.int32, device="cuda")
+        g = metset.solve(blk, kxy).cpu().numpy()
+        g = g[:, 0] + 1j * g[:, 1]
+        assert rel_l2(g, g_or) < 1e-8
+
+
+def _ws_views(ws, P):
+    import torch
+
+    NIN, W, NL = 96, 256, 5
+    b = ws.view(torch.bfloat16)
+    x0 = b[: NIN * P].view(NIN, P).float().cpu().numpy().T.astype(np.float64)
+    return x0
+
+
+def test_forward_and_gradient_match_numpy(metset, net):
+    import torch
+
+    from bldfm_amd import surrogate as sur
+
+    # a non-zero output layer so every gradient block is exercised
+    rng = np.random.default_rng(11)
+    p = net.params.cpu().numpy().astype(np.float64)
+    off, n = osur.layout()
+    o, s = off["Wo"]
+    p[o : o + 512] = rng.normal(0.0, 0.05, 512)
+    p[off["bo"][0] : off["bo"][0] + 2] = [0.3, -0.2]
+    net.params.copy_(torch.from_numpy(p.astype(np.float32)).cuda())
+    net.pack()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    mids, kxy = net.sample_points(metset, 4, gen)  # 1024 points, 4 met samples
+    g = metset.solve(mids, kxy)
+    P = kxy.shape[0]
+    ws = torch.zeros(P * net.bytes_per_point, dtype=torch.uint8, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    phi = torch.empty((P, 2), dtype=torch.float32, device="cuda")
+    net.grad.zero_()
+    blk = mids.repeat_interleave(2)
+    net.grad_step(metset, blk, kxy, g, ws=ws, loss=loss, phi=phi)
+    torch.cuda.synchronize()
+    X0 = _ws_views(ws, P)
+    # inputs: the kernel's features against the numpy features (1 bf16 ulp,
+    # plus the fp32 rounding of RFF arguments up to ~60 rad)
+    sm = metset.surmet_host[np.repeat(mids.cpu().numpy(), 256)]
+    k = kxy.cpu().numpy()
+    Xr = osur.features(net.static.numpy(), sm, k[:, 0], k[:, 1])
+    assert np.all(np.abs(X0 - osur.bf16(Xr)) <= 2.0 ** -7 * np.abs(Xr) + 1e-4)
+    # forward, loss, gradient against numpy with the kernels' roundings
+    p0 = osur.phi0(sm, k[:, 0], k[:, 1])
+    L, grad, phi_r = osur.loss_grad(p, X0, g.cpu().numpy(), p0, block=128, bf16_mode=True)
+    assert rel_l2(phi.cpu().numpy(), phi_r) < 1e-3
+    assert abs(float(loss.item()) - L) <= 1e-3 * abs(L)
+    gk = net.grad.cpu().numpy().astype(np.float64)
+    for name, (o, s) in off.items():
+        sz = int(np.prod(s))
+        assert rel_l2(gk[o : o + sz], grad[o : o + sz]) < 1e-2, name
+    # inference entry point = training forward
+    phi2 = net.phi(metset, blk, kxy)
+    assert torch.equal(phi2, phi)
+
+
+def test_adam_matches_numpy(net):
+    import torch
+
+    rng = np.random.default_rng(2)
+    p0 = net.params.cpu().numpy().astype(np.float64)
+    g = rng.normal(0.0, 1e-3, p0.size)
+    net.grad.copy_(torch.from_numpy(g.astype(np.float32)).cuda())
+    m0 = net.m.cpu().numpy().astype(np.float64)
+    v0 = net.v.cpu().numpy().astype(np.float64)
+    t = net.step + 1
+    net.adam(1e-3)
+    torch.cuda.synchronize()
+    m = 0.9 * m0 + 0.1 * g
+    v = 0.999 * v0 + 0.001 * g * g
+    p = p0 - 1e-3 * (m / (1 - 0.9**t)) / (np.sqrt(v / (1 - 0.999**t)) + 1e-8)
+    assert np.allclose(net.params.cpu().numpy(), p, rtol=1e-5, atol=1e-8)
+    assert float(net.grad.abs().max()) == 0.0
+
+
+def _specs(zms, L=(-20.0, 100.0)):
+    from bldfm_amd.batch import FootprintSpec
+    from bldfm_amd.pbl_model import psi
+
+    specs = []
+    for i, zm in enumerate(zms):
+        mol = L[i % len(L)]
+        speed = 0.3 / 0.4 * (np.log(zm / 0.05) + float(psi(zm / mol)))
+        wd = np.deg2rad(30.0 + 70.0 * i)
+        specs.append(FootprintSpec(nx=128, ny=128, domain=(500.0, 500.0), modes=(128, 128), meas_pt=(250.0, 250.0),
+                                   n=26, meas_height=zm, wind=(-speed * np.sin(wd), -speed * np.cos(wd)), ustar=0.3,
+                                   mol=mol, domain_height=zm * 100.0 / 15.0))
+    return specs
+
+
+def test_gate_falls_back_to_the_solver(net):
+    """An untrained network (the WKB prior alone) fails every spot check, and an
+    instance outside the hull is never served: all outputs equal the plain
+    FP64 batch bit for bit."""
+    import torch
+
+    from bldfm_amd import surrogate as sur
+    from bldfm_amd.batch import FootprintBatch
+
+    untrained = sur.SurrogateNet(net.static.numpy()[132:140], net.static.numpy()[140:148], seed=1)
+    specs = _specs([10.0, 15.0, 60.0])  # the last is above the hull's zm range
+    plain = FootprintBatch(specs)
+    plain.run()
+    fb = FootprintBatch(specs, surrogate=untrained, tol=1e-3)
+    fb.run()
+    torch.cuda.synchronize()
+    err, mode = fb.surrogate_stats()
+    assert mode == [1, 1, 1]
+    assert err[0] > 1e-3 and err[1] > 1e-3 and np.isnan(err[2])
+    for i in range(len(specs)):
+        for a, b in zip(fb.output(i), plain.output(i)):
+            assert torch.equal(a, b)
+
+
+def test_gate_accepts_within_tolerance(net):
+    """With a tolerance above the untrained network's error every in-hull
+    instance is served by the network (conc 0), and the spot-check estimate
+    is of the size of the whole footprint's relative L2 error."""
+    import torch
+
+    from bldfm_amd import surrogate as sur
+    from bldfm_amd.batch import FootprintBatch
+
+    untrained = sur.SurrogateNet(net.static.numpy()[132:140], net.static.numpy()[140:148], seed=1)
+    specs = _specs([10.0, 15.0])
+    plain = FootprintBatch(specs)
+    plain.run()
+    fb = FootprintBatch(specs, surrogate=untrained, tol=1e9)
+    fb.run()
+    torch.cuda.synchronize()
+    err, mode = fb.surrogate_stats()
+    assert mode == [0, 0]
+    for i in range(2):
+        conc, flx = fb.output(i)
+        assert float(conc.abs().max()) == 0.0
+        f = flx[0].cpu().numpy()
+        assert np.all(np.isfinite(f))
+        e = rel_l2(f, plain.output(i)[1][0].cpu().numpy())
+        assert err[i] > 0.1 and e > 0.1
+        assert e / 3.0 < err[i] < 3.0 * e
