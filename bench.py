@@ -16,6 +16,14 @@ events on the launch stream, and a CPU baseline: the numpy oracle timed on
 a bounded sample of the same workload on one host core.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+                    [--surrogate CHECKPOINT] [--tol 1e-3] [--no-surrogate]
+
+The surrogate leg serves the same batch with the trained network
+(``bldfm_amd/data``), re-solving 1 % of every footprint's modes in FP64 as
+an error gate and solving a footprint in FP64 when that check exceeds
+``--tol``; it reports its footprints/s, the accepted/fallback split, the
+footprint and footprint-integral errors against the FP64 batch and the
+network kernel's bf16 MFMA throughput.
 """
 
 import argparse
@@ -95,6 +103,66 @@ def timed(batch_obj, steps, warmup):
     return t1 - t0, phases, n
 
 
+MFMA_FLOP_PER_POINT = 2 * (96 * 256 + 4 * 256 * 256)  # bf16 MFMA work of one network evaluation
+PEAK_BF16 = 2.5e15  # MI355X dense bf16 MFMA
+
+
+def surrogate_leg(specs, fb, args):
+    """The same batch served by the trained surrogate with the 1 % FP64
+    spot-check gate and FP64 fallback at rel-L2 > tol; footprint errors
+    against the exact batch ``fb`` (same specs, already run)."""
+    import torch
+
+    from bldfm_amd import surrogate as sur
+    from bldfm_amd.batch import FootprintBatch
+
+    if not os.path.exists(args.surrogate):
+        return {"error": f"no trained network at {args.surrogate}"}
+    net = sur.SurrogateNet.load(args.surrogate)
+    fbs = FootprintBatch(specs, precision="double", fast=False, surrogate=net, tol=args.tol)
+    elapsed, phases, n = timed(fbs, args.steps, args.warmup)
+    fps = len(specs) * args.steps / elapsed
+    err, mode = fbs.surrogate_stats()
+    rel, integ = [], []
+    for i in range(len(specs)):
+        a, b = fbs.output(i)[1], fb.output(i)[1]
+        rel.append(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
+        integ.append(abs(float(a.sum() - b.sum())) / abs(float(b.sum())))
+    acc = [i for i, m in enumerate(mode) if m == 0]
+    points = sum(-(-l // 128) * 128 for l, _ in fbs.lanes_and_steps())
+    sur_ms = phases["surrogate"] / max(n, 1)
+    achieved = MFMA_FLOP_PER_POINT * points / (sur_ms * 1e-3) if acc else None
+    del fbs
+    pick = lambda v, idx: [v[i] for i in idx]
+    return {
+        "value": round(fps, 2),
+        "unit": "footprints/s",
+        "ratio_vs_fp64_solver": None,  # filled by the caller
+        "tol": args.tol,
+        "spot_check": "1 mode in 100 re-solved in FP64 per footprint",
+        "accepted": len(acc),
+        "fallback": len(specs) - len(acc),
+        "footprint_rel_l2_accepted": {"max": max(pick(rel, acc)) if acc else None,
+                                      "median": float(np.median(pick(rel, acc))) if acc else None},
+        "footprint_rel_l2_all": {"max": max(rel), "median": float(np.median(rel))},
+        "spot_estimate_median": float(np.nanmedian(err)),
+        "footprint_integral_rel_err_accepted": {"max": max(pick(integ, acc)) if acc else None,
+                                                "median": float(np.median(pick(integ, acc))) if acc else None},
+        "phase_ms_per_step": {k: round(v / max(n, 1), 4) for k, v in phases.items()},
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "k_sur_plan (bf16 MFMA network on every mode) + spot check + gate",
+            "achieved": round(achieved / 1e12, 2) if achieved else None,
+            "peak": PEAK_BF16 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_BF16, 4) if achieved else None,
+            "flop_per_point": MFMA_FLOP_PER_POINT,
+            "points_per_launch": points,
+        },
+        "network": os.path.relpath(args.surrogate, ROOT),
+    }
+
+
 def cpu_baseline(specs, gpu_batch, nsample):
     """Oracle (numpy restatement of the reference algorithm) on one core, on
     the first ``nsample`` footprints of the workload; also returns the
@@ -132,6 +200,9 @@ def main():
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-fast", action="store_true")
+    ap.add_argument("--surrogate", default=os.path.join(ROOT, "bldfm_amd", "data", "surrogate_most_n26.pt"))
+    ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -168,6 +239,14 @@ def main():
                 d = max(d, float(np.linalg.norm(a - b) / np.linalg.norm(b)))
         extra["fast_fma_rel_l2_vs_exact"] = d
         del fbf
+
+    if not args.no_surrogate:
+        sl = surrogate_leg(specs, fb, args)
+        if "value" in sl:
+            sl["ratio_vs_fp64_solver"] = round(sl["value"] / fps, 4)
+            print(f"[bench] surrogate: {sl['value']:.1f} footprints/s ({sl['ratio_vs_fp64_solver']:.3f}x the "
+                  f"FP64 solver), {sl['accepted']} accepted / {sl['fallback']} fallback", file=sys.stderr, flush=True)
+        extra["surrogate"] = sl
 
     cpu_fps, worst = cpu_baseline(specs, fb, args.cpu_sample) if args.cpu_sample > 0 else (None, None)
     out = {

@@ -236,14 +236,14 @@ class Plan:
             "plan_execute",
         )
 
-    PHASES = ("level_table", "forward_fft", "shoot", "nyquist_fix", "inverse_y", "inverse_x")
+    PHASES = ("level_table", "forward_fft", "surrogate", "shoot", "nyquist_fix", "inverse_y", "inverse_x")
 
     def timing_begin(self, max_execs):
         check(load().bldfm_plan_timing_begin(self.handle, int(max_execs)), "timing_begin")
 
     def timing_end(self):
         """Summed milliseconds per phase over the timed executions, and their count."""
-        ms = (ctypes.c_double * 6)()
+        ms = (ctypes.c_double * len(self.PHASES))()
         n = _I32()
         check(load().bldfm_plan_timing_end(self.handle, ms, ctypes.byref(n)), "timing_end")
         return dict(zip(self.PHASES, list(ms))), n.value

@@ -560,7 +560,7 @@ int bldfm_plan_execute(bldfm_plan* P, double* prof, const double* src, void* con
   B.gate = sur ? B.ws + P->off_gate : nullptr;
   B.mode = sur ? (int32_t*)(B.ws + P->off_mode) : nullptr;
   hipEvent_t* ev = nullptr;
-  if (P->timing_count < P->timing_max) ev = P->events.data() + 7 * (P->timing_count++);
+  if (P->timing_count < P->timing_max) ev = P->events.data() + (BLDFM_PHASES + 1) * (P->timing_count++);
   const int rc = bldfm_launch_all(P->dev, B, kGridCap, stream, ev);
   if (rc == -1999) return fail(BLDFM_EHIP, "hipEventRecord failed");
   if (rc) return fail(BLDFM_EHIP, std::string("kernel launch failed: ") +
@@ -572,7 +572,7 @@ int bldfm_plan_timing_begin(bldfm_plan* P, int32_t max_execs) {
   if (!P || max_execs <= 0 || max_execs > 100000) return fail(BLDFM_EINVAL, "bad timing arguments");
   DeviceGuard guard(P->device);
   destroy_events(P);
-  P->events.resize((size_t)7 * max_execs, nullptr);
+  P->events.resize((size_t)(BLDFM_PHASES + 1) * max_execs, nullptr);
   for (auto& e : P->events) {
     hipError_t err = hipEventCreate(&e);
     if (err != hipSuccess) { destroy_events(P); return hip_fail(err, "hipEventCreate"); }
@@ -585,16 +585,17 @@ int bldfm_plan_timing_begin(bldfm_plan* P, int32_t max_execs) {
 int bldfm_plan_timing_end(bldfm_plan* P, double* phase_ms, int32_t* nexecs) {
   if (!P || !phase_ms) return fail(BLDFM_EINVAL, "bad timing arguments");
   DeviceGuard guard(P->device);
-  for (int k = 0; k < 6; ++k) phase_ms[k] = 0.0;
+  for (int k = 0; k < BLDFM_PHASES; ++k) phase_ms[k] = 0.0;
   const int n = P->timing_count;
   int rc = 0;
   if (n > 0) {
-    hipError_t err = hipEventSynchronize(P->events[7 * (n - 1) + 6]);
+    hipError_t err = hipEventSynchronize(P->events[(BLDFM_PHASES + 1) * (n - 1) + BLDFM_PHASES]);
     if (err != hipSuccess) rc = hip_fail(err, "hipEventSynchronize");
     for (int x = 0; x < n && !rc; ++x)
-      for (int k = 0; k < 6; ++k) {
+      for (int k = 0; k < BLDFM_PHASES; ++k) {
         float ms = 0.f;
-        err = hipEventElapsedTime(&ms, P->events[7 * x + k], P->events[7 * x + k + 1]);
+        const int e0 = (BLDFM_PHASES + 1) * x + k;
+        err = hipEventElapsedTime(&ms, P->events[e0], P->events[e0 + 1]);
         if (err != hipSuccess) { rc = hip_fail(err, "hipEventElapsedTime"); break; }
         phase_ms[k] += ms;
       }

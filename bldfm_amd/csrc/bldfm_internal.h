@@ -51,8 +51,9 @@ struct ExecBufs {
   int32_t* mode;       // ninst: 0 surrogate accepted, 1 FP64 solver; null: every instance solved
 };
 
-// ev (optional): 7 events recorded at the phase boundaries
-// [level table | forward transform | shooting | Nyquist fix-up | y-pass | x-pass]
+// ev (optional): BLDFM_PHASES + 1 events recorded at the phase boundaries
+// [level table | forward transform | surrogate and gate | shooting | Nyquist fix-up | y-pass | x-pass]
+#define BLDFM_PHASES 7
 int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStream_t stream,
                      hipEvent_t* ev = nullptr);
 // Surrogate inference over the lanes of accepted instances (packed spectrum).

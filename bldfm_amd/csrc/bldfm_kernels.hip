@@ -578,6 +578,7 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
     k_sur_gate<<<gi, 256, 0, stream>>>(P, B);
     LAUNCH_CHECK();
   }
+  MARK(3)
   const unsigned gs = grid_for(P.tot_shoot, grid_cap);
 #define SHOOT(V, MAXS, RECOMP, EX)                                                    \
   if (P.variant_used[V][EX]) {                                                        \
@@ -595,16 +596,16 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
     k_shoot_analytic<<<gs, SHOOT_THREADS, 0, stream>>>(P, B, 3);
     LAUNCH_CHECK();
   }
-  MARK(3)
+  MARK(4)
   if (P.tot_nyq > 0) {
     k_nyq_fix<<<grid_for(P.tot_nyq, grid_cap), 256, 0, stream>>>(P, B);
     LAUNCH_CHECK();
   }
-  MARK(4)
-  FFT_LAUNCH(k_inv_cols, P.tot_inv_cols)
   MARK(5)
-  FFT_LAUNCH(k_inv_rows, P.tot_inv_rows)
+  FFT_LAUNCH(k_inv_cols, P.tot_inv_cols)
   MARK(6)
+  FFT_LAUNCH(k_inv_rows, P.tot_inv_rows)
+  MARK(7)
 #undef FFT_LAUNCH
 #undef MARK
   return 0;

@@ -115,7 +115,7 @@ __device__ __forceinline__ void sur_features(cstat_t& S, cmet_t& M, float lx, fl
 // KB16 steps of 16, W in forward fragment order (sur_pack_fwd / sur_pack_bwd).
 // Element j of acc[a][pb] in lane (half, r) is output feature
 // 64 wave + 32 a + 8 (j/4) + 4 half + j%4 of point 32 pb + r.
-template <int KB16>
+template <int KB16, int UNROLL>
 __device__ __forceinline__ void mfma_tile(const bf16x8* __restrict__ Wp, const __bf16* act, int wave,
                                           int lane, f32x16 (&acc)[2][4]) {
 #pragma unroll
@@ -127,7 +127,9 @@ __device__ __forceinline__ void mfma_tile(const bf16x8* __restrict__ Wp, const _
   const int r = lane & 31, half = lane >> 5;
   const bf16x8* A0p = Wp + (int64_t)(wave * 2 + 0) * KB16 * 64 + lane;
   const bf16x8* A1p = Wp + (int64_t)(wave * 2 + 1) * KB16 * 64 + lane;
-#pragma unroll 1
+  // weight fragments come from L2: unrolled, the scheduler issues later
+  // steps' loads under earlier steps' MFMAs (UNROLL = KB16 where registers allow)
+#pragma unroll UNROLL
   for (int kb = 0; kb < KB16; ++kb) {
     const bf16x8 A0 = A0p[kb * 64];
     const bf16x8 A1 = A1p[kb * 64];
@@ -163,10 +165,11 @@ __device__ __forceinline__ void sur_hidden(const uint8_t* __restrict__ net, __bf
   const SurDyn* __restrict__ dyn = (const SurDyn*)(net + SUR_OFF_DYN);
   f32x16 acc[2][4];
   for (int l = 0; l < SUR_NL; ++l) {
+    constexpr int U = TRAIN ? 1 : SUR_W / 16;  // the training kernel has no registers to spare
     if (l == 0)
-      mfma_tile<SUR_IN / 16>((const bf16x8*)(net + SUR_OFF_W0P), act, wave, lane, acc);
+      mfma_tile<SUR_IN / 16, TRAIN ? 1 : SUR_IN / 16>((const bf16x8*)(net + SUR_OFF_W0P), act, wave, lane, acc);
     else
-      mfma_tile<SUR_W / 16>((const bf16x8*)(net + SUR_OFF_WP(l)), act, wave, lane, acc);
+      mfma_tile<SUR_W / 16, U>((const bf16x8*)(net + SUR_OFF_WP(l)), act, wave, lane, acc);
     __syncthreads();  // every wave has read this layer's input tile
     const bool last = (l == SUR_NL - 1);
     float po[4][2];
@@ -241,8 +244,8 @@ __device__ __forceinline__ void sur_hidden(const uint8_t* __restrict__ net, __bf
 // block's solver energy S = sum |G|^2 and by the number of blocks:
 //   sum_i |G_i - G^_i|^2 + |G_i|^2 |Phi^_i + log G_i|^2   (imaginary part wrapped to [-pi, pi])
 // with G^ = exp(-Phi^): the footprint's L2 error plus a log-space term that
-// keeps the phase well conditioned on the energetic modes.  Points with
-// |G| > 1.5 or a non-finite G are left out.
+// keeps the phase well conditioned on the energetic modes.  Points whose
+// G is roundoff (see below) are left out.
 template <bool TRAIN>
 __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_fwd(const uint8_t* __restrict__ net,
                                                          const SurMet* __restrict__ mets, int nmet,
@@ -271,15 +274,17 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_fwd(const uint8_t* __res
   if (phi && tid < SUR_PTS) phi[p0 + tid] = phis[tid];
   if (!TRAIN) return;
 
-  // Points whose solver transfer is not a physical one (|G| > 1.5 or not
-  // finite: the shooting's cancellation fails for very deep columns) carry
-  // no loss.
+  // Points whose solver transfer is roundoff rather than physics carry no
+  // loss: not finite, |G| > 1.5, or |G| above 100 times the WKB prior (plus
+  // 1e-8) -- in deep, stable columns the shooting's cancellation leaves
+  // noise where G is exponentially small.
   float gr = 0.f, gi = 0.f, s2 = 0.f;
   bool valid = false;
   if (tid < SUR_PTS) {
     const double2 g = gt[p0 + tid];
     const double a2 = g.x * g.x + g.y * g.y;
-    valid = a2 <= 2.25;  // false for NaN
+    const double b = 100.0 * exp(-(double)p0s[tid].x) + 1e-8;
+    valid = a2 <= 2.25 && a2 <= b * b;  // false for NaN
     if (valid) {
       gr = (float)g.x;
       gi = (float)g.y;
@@ -396,7 +401,7 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_bwd(const uint8_t* __res
   bias_grad(act, grad + SUR_P_B(SUR_NL - 1));
   f32x16 acc[2][4];
   for (int l = SUR_NL - 1; l >= 1; --l) {
-    mfma_tile<SUR_W / 16>((const bf16x8*)(net + SUR_OFF_WT(l)), act, wave, lane, acc);
+    mfma_tile<SUR_W / 16, SUR_W / 16>((const bf16x8*)(net + SUR_OFF_WT(l)), act, wave, lane, acc);
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < 2; ++a) {

@@ -154,12 +154,14 @@ class MetSet:
         self.surmet_host = sm
         self.surmet = torch.from_numpy(sm).to(self.device)
 
-    def conditioned(self, kmax, probe=32, bound=1.0 + 1e-6):
-        """Samples whose FP64 transfer is physical (|G| <= bound) on a
-        probe x probe lattice of [-kmax, kmax]^2.  In deep, stable columns
-        the shooting's cancellation loses every digit at large |k| (the
-        reference's method does the same); such samples are no training
-        data."""
+    def conditioned(self, kmax, probe=32):
+        """Samples whose FP64 transfer is physics rather than roundoff on a
+        probe x probe lattice of [-kmax, kmax]^2: finite, |G| <= 1 + 1e-6
+        and |G| <= 100 |exp(-Phi0)| + 1e-8 with the WKB prior Phi0.  In
+        deep, stable columns the shooting's cancellation (the reference's
+        method, reproduced here) leaves noise where G is exponentially
+        small; such samples are no training data, and at inference the
+        spot check sends them to the solver."""
         k = np.linspace(-kmax, kmax, probe)
         kx, ky = np.meshgrid(k, k)
         pts = np.stack([kx.ravel(), ky.ravel()], 1)
@@ -169,9 +171,11 @@ class MetSet:
         kxy = torch.from_numpy(np.tile(pad, (self.n, 1))).to(self.device)
         blk = torch.arange(self.n, dtype=torch.int32, device=self.device).repeat_interleave(per)
         g = self.solve(blk, kxy).view(self.n, per * TRANSFER_BLOCK, 2)
-        a2 = (g * g).sum(dim=2)
-        ok = torch.where(torch.isfinite(a2), a2, torch.full_like(a2, 1e300)).max(dim=1).values <= bound * bound
-        return ok.cpu().numpy()
+        prior = SurrogateNet(np.zeros(N_MET), np.ones(N_MET), hull=self.hull, dev=self.device)  # untrained: Phi0
+        p0 = prior.phi(self, blk.repeat_interleave(2), kxy)[:, 0].double().view(self.n, -1)
+        a = (g * g).sum(dim=2).sqrt()
+        bad = ~torch.isfinite(a) | (a > 1.0 + 1e-6) | (a > 100.0 * torch.exp(-p0) + 1e-8)
+        return (~bad.any(dim=1)).cpu().numpy()
 
     def subset(self, mask):
         """A MetSet of the samples where mask is true."""
@@ -360,15 +364,20 @@ class SurrogateNet:
         return np.asarray(errs)
 
     # ------------------------------------------------------------ checkpoint
-    def state(self):
-        return {"params": self.params.cpu(), "m": self.m.cpu(), "v": self.v.cpu(),
-                "step": torch.tensor(self.step), "static": self.static.clone(),
-                "hull": torch.tensor([*self.hull.ustar, *self.hull.invl, *self.hull.z0, self.hull.z0_zm_max,
-                                      *self.hull.zm, self.hull.kmax, self.hull.top_ratio, self.hull.n],
-                                     dtype=torch.float64)}
+    def state(self, with_optimizer=True):
+        st = {"params": self.params.cpu(), "step": torch.tensor(self.step), "static": self.static.clone(),
+              "hull": torch.tensor([*self.hull.ustar, *self.hull.invl, *self.hull.z0, self.hull.z0_zm_max,
+                                    *self.hull.zm, self.hull.kmax, self.hull.top_ratio, self.hull.n],
+                                   dtype=torch.float64)}
+        if with_optimizer:
+            st["m"] = self.m.cpu()
+            st["v"] = self.v.cpu()
+        return st
 
-    def save(self, path):
-        torch.save(self.state(), path)
+    def save(self, path, with_optimizer=True):
+        """Checkpoint: fp32 master parameters, the fixed feature header, the
+        hull and (to resume training) the Adam moments and step."""
+        torch.save(self.state(with_optimizer), path)
 
     @classmethod
     def load(cls, path, dev=None):
@@ -381,8 +390,9 @@ class SurrogateNet:
         net.static = st["static"].clone()
         net.net[: 4 * STATIC_FLOATS].copy_(net.static.view(torch.uint8).to(net.device))
         net.params.copy_(st["params"].to(net.device))
-        net.m.copy_(st["m"].to(net.device))
-        net.v.copy_(st["v"].to(net.device))
+        if "m" in st:
+            net.m.copy_(st["m"].to(net.device))
+            net.v.copy_(st["v"].to(net.device))
         net.step = int(st["step"])
         net.pack()
         return net
@@ -395,3 +405,39 @@ def met_normalisation(metset: MetSet):
     sd = sm.std(axis=0)
     sd[sd < 1e-6] = 1.0
     return mu.astype(np.float32), sd.astype(np.float32)
+
+
+DEFAULT_CHECKPOINT = __import__("pathlib").Path(__file__).resolve().parent / "data" / "surrogate_most_n26.pt"
+
+
+def train_default(steps=400000, pool=22000, seed=1, heldout=48, out=None, log=print):
+    """The shipped network: ``pool`` met samples of the default hull (those
+    the FP64 solver resolves), ``steps`` Adam steps of 65,536 fresh points
+    each, lr 1e-3 with cosine decay; reports the held-out footprint error
+    of the prior alone and of the trained network."""
+    ms = MetSet(sample_met(pool, seed=seed))
+    ms = ms.subset(ms.conditioned(ms.hull.kmax))
+    ho = MetSet(sample_met(heldout, seed=seed + 1))
+    ho = ho.subset(ho.conditioned(ho.hull.kmax))
+    mu, sd = met_normalisation(ms)
+    net = SurrogateNet(mu, sd, seed=0)
+    prior = net.evaluate(ho)
+    log(f"training pool {ms.n} samples, held out {ho.n}; prior alone: median {np.median(prior):.3e}")
+    tl = net.train(ms, steps, log_every=max(steps // 20, 1),
+                   callback=lambda it, l: log(f"  step {it} loss {l:.4e}"))
+    err = net.evaluate(ho)
+    log(f"{steps} steps in {tl.seconds:.1f} s; held-out footprint rel-L2 median {np.median(err):.3e} "
+        f"p90 {np.percentile(err, 90):.3e} max {err.max():.3e}")
+    if out is not None:
+        net.save(out, with_optimizer=False)
+    return net, prior, err
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="Train the footprint-transfer surrogate on the GPU")
+    ap.add_argument("--steps", type=int, default=400000)
+    ap.add_argument("--out", default=str(DEFAULT_CHECKPOINT))
+    a = ap.parse_args()
+    train_default(a.steps, out=a.out, log=lambda m: print(m, flush=True))

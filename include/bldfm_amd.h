@@ -164,9 +164,10 @@ int bldfm_plan_execute(bldfm_plan* plan, double* prof, const double* src, void* 
  * Phase timing (no counterpart in the reference; for benchmarks): after
  * bldfm_plan_timing_begin, the next max_execs executions record HIP events
  * on their stream at the phase boundaries; bldfm_plan_timing_end waits for
- * them and returns the summed milliseconds of the six phases
- * [level table, forward transform, shooting, Nyquist fix-up, y-pass, x-pass]
- * and the number of timed executions.  Not for use under graph capture.
+ * them and returns the summed milliseconds of the seven phases
+ * [level table, forward transform, surrogate and its gate, shooting,
+ * Nyquist fix-up, y-pass, x-pass] and the number of timed executions.
+ * Not for use under graph capture.
  */
 int bldfm_plan_timing_begin(bldfm_plan* plan, int32_t max_execs);
 int bldfm_plan_timing_end(bldfm_plan* plan, double* phase_ms, int32_t* nexecs);

@@ -123,16 +123,19 @@ def forward(p, X0, bf16_mode=True):
     return phi, xs, zs
 
 
-def loss_and_dphi(phi, g, block=128):
+def loss_and_dphi(phi, g, block=128, p0=None):
     """The training loss of the kernels and dLoss/dPhi.  Per block of
     ``block`` points (one met sample) with S = sum |G|^2:
     sum_i |G_i - exp(-Phi_i)|^2 + |G_i|^2 |Phi_i + log G_i|^2 (imaginary
     part wrapped to [-pi, pi]), divided by S and the number of blocks;
-    points with |G| > 1.5 or a non-finite G are left out."""
+    points with |G| > 1.5, a non-finite G, or (given the prior p0)
+    |G| > 100 exp(-Re p0) + 1e-8 are left out."""
     P = phi.shape[0]
     nb = P // block
     G = g[:, 0] + 1j * g[:, 1]
     valid = np.abs(G) ** 2 <= 2.25  # False for NaN
+    if p0 is not None:  # roundoff, not physics: far above the WKB prior
+        valid &= np.abs(G) <= 100.0 * np.exp(-p0[:, 0]) + 1e-8
     G = np.where(valid, G, 0.0)
     s2 = np.abs(G) ** 2
     mag = np.exp(-np.maximum(phi[:, 0], -20.0))
@@ -173,7 +176,7 @@ def loss_grad(flat, X0, g, p0=None, block=128, bf16_mode=True):
     p = unpack(flat)
     out, xs, zs = forward(p, X0, bf16_mode)
     phi = out + (0.0 if p0 is None else p0)
-    L, dphi = loss_and_dphi(phi, g, block)
+    L, dphi = loss_and_dphi(phi, g, block, p0)
     return L, pack(backward(p, xs, zs, dphi, bf16_mode)), phi
 
 

@@ -208,3 +208,60 @@ def test_gate_accepts_within_tolerance(net):
         e = rel_l2(f, plain.output(i)[1][0].cpu().numpy())
         assert err[i] > 0.1 and e > 0.1
         assert e / 3.0 < err[i] < 3.0 * e
+
+
+@pytest.fixture(scope="module")
+def trained():
+    from bldfm_amd import surrogate as sur
+
+    if not sur.DEFAULT_CHECKPOINT.exists():
+        pytest.skip("no trained network shipped")
+    return sur.SurrogateNet.load(sur.DEFAULT_CHECKPOINT)
+
+
+def test_trained_network_gain_over_the_prior(trained):
+    """On held-out met samples (a seed not used in training) the trained
+    network's footprint error is below 1e-3 where the WKB prior alone (the
+    method without the network) is off by tens of percent."""
+    import torch
+
+    from bldfm_amd import surrogate as sur
+
+    ho = sur.MetSet(sur.sample_met(12, seed=99))
+    ho = ho.subset(ho.conditioned(trained.hull.kmax))
+    assert ho.n >= 6
+    e = trained.evaluate(ho)
+    static = trained.static.numpy()
+    prior = sur.SurrogateNet(static[132:140], static[140:148], hull=trained.hull)
+    prior.static = trained.static.clone()
+    prior.net[: 4 * sur.STATIC_FLOATS].copy_(prior.static.view(torch.uint8).cuda())
+    e0 = prior.evaluate(ho)
+    assert np.median(e) < 1e-3 and e.max() < 3e-3
+    assert np.median(e0) > 0.1
+    assert np.median(e0) / np.median(e) > 100.0
+
+
+def test_trained_surrogate_batch_within_tolerance(trained):
+    """Served instances' flux footprints are within 1.5 tol of the FP64
+    batch (the gate's 1 % estimate decides at tol); fallbacks are bitwise
+    the FP64 footprint."""
+    import torch
+
+    from bldfm_amd.batch import FootprintBatch
+
+    specs = _specs([8.0, 12.0, 20.0, 25.0], L=(-20.0, -100.0, 100.0, -50.0))
+    plain = FootprintBatch(specs)
+    plain.run()
+    tol = 1e-3
+    fb = FootprintBatch(specs, surrogate=trained, tol=tol)
+    fb.run()
+    torch.cuda.synchronize()
+    err, mode = fb.surrogate_stats()
+    assert sum(1 for m in mode if m == 0) >= 2, (err, mode)
+    for i in range(len(specs)):
+        a, b = fb.output(i)[1], plain.output(i)[1]
+        if mode[i] == 0:
+            assert err[i] <= tol
+            assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1.5 * tol
+        else:
+            assert torch.equal(a, b)
