@@ -30,12 +30,16 @@ typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef const __attribute__((address_space(4))) SurStatic cstat_t;
 typedef const __attribute__((address_space(4))) SurMet cmet_t;
 
-#define SUR_TWO_PI 6.283185307179586f
+#define SUR_TWO_PI 6.283185307179586f  // loss phase wrap
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware exp2 and reciprocal (about one
+// ulp each; the results are rounded to bf16 or feed bf16 products).
+__device__ __forceinline__ float sigmoid_f(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+__device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
 
 __device__ __forceinline__ float dsilu_f(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid_f(z);
   return s + z * s * (1.0f - s);
 }
 
@@ -92,12 +96,12 @@ __device__ __forceinline__ void sur_features(cstat_t& S, cmet_t& M, float lx, fl
   };
   const int j0 = part * (SUR_NRFF / 2);
   for (int j = j0; j < j0 + SUR_NRFF / 2; ++j) {
-    const float th = SUR_TWO_PI * (S.rff[j][0] * c[0] + S.rff[j][1] * c[1] + S.rff[j][2] * c[2] +
-                                   S.rff[j][3] * c[3]);
-    float sn, cs;
-    sincosf(th, &sn, &cs);
-    put(j, sn);
-    put(SUR_NRFF + j, cs);
+    // phase in revolutions, reduced exactly to [-1/2, 1/2]; the hardware
+    // sine and cosine take revolutions
+    float rev = S.rff[j][0] * c[0] + S.rff[j][1] * c[1] + S.rff[j][2] * c[2] + S.rff[j][3] * c[3];
+    rev -= rintf(rev);
+    put(j, __builtin_amdgcn_sinf(rev));
+    put(SUR_NRFF + j, __builtin_amdgcn_cosf(rev));
   }
   if (part == 1) {
     for (int d = 0; d < SUR_NC; ++d) put(2 * SUR_NRFF + d, c[d]);
@@ -154,15 +158,23 @@ __device__ __forceinline__ void lds_put4(__bf16* act, int pt, int o0, const floa
   *(bf16x4*)(act + sur_act_off(pt, o0)) = w;
 }
 
-// Hidden layers and output of the points whose inputs are in `act`.
-// Leaves Phi = Phi0 + network output of local point t (t < SUR_PTS) in phis[t].  TRAIN: stores z_l
+// Hidden layers and output of the points whose inputs are in `act`, with
+// biases and output weights from `dyn` (LDS).  Leaves Phi = Phi0 + network
+// output of local point t (t < SUR_PTS) in phis[t].  TRAIN: stores z_l
 // and x_{l+1} (bf16) feature-major at point offset p0.
+// Copy of the network's biases and output layer into LDS (dyn), read by the
+// epilogues instead of L2.
+__device__ __forceinline__ void stage_dyn(const uint8_t* __restrict__ net, SurDyn* dsh) {
+  const float4* src = (const float4*)(net + SUR_OFF_DYN);
+  for (int i = threadIdx.x; i < (int)(sizeof(SurDyn) / 16); i += blockDim.x) ((float4*)dsh)[i] = src[i];
+}
+
 template <bool TRAIN>
-__device__ __forceinline__ void sur_hidden(const uint8_t* __restrict__ net, __bf16* act, float* red,
-                                           const float2* p0s, float2* phis, __bf16* __restrict__ xs,
-                                           __bf16* __restrict__ zs, int64_t N, int64_t p0) {
+__device__ __forceinline__ void sur_hidden(const uint8_t* __restrict__ net, const SurDyn* dyn, __bf16* act,
+                                           float* red, const float2* p0s, float2* phis,
+                                           __bf16* __restrict__ xs, __bf16* __restrict__ zs, int64_t N,
+                                           int64_t p0) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 31, half = lane >> 5;
-  const SurDyn* __restrict__ dyn = (const SurDyn*)(net + SUR_OFF_DYN);
   f32x16 acc[2][4];
   for (int l = 0; l < SUR_NL; ++l) {
     constexpr int U = TRAIN ? 1 : SUR_W / 16;  // the training kernel has no registers to spare
@@ -259,6 +271,7 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_fwd(const uint8_t* __res
   __shared__ float red[4 * SUR_PTS * 2];
   __shared__ float2 phis[SUR_PTS];
   __shared__ float2 p0s[SUR_PTS];
+  __shared__ __attribute__((aligned(16))) SurDyn dsh;
   __shared__ float wsum[8];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t p0 = (int64_t)blockIdx.x * SUR_PTS;
@@ -269,8 +282,9 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_fwd(const uint8_t* __res
   const int pt = tid & (SUR_PTS - 1), part = tid >> 7;
   const double2 k = kxy[p0 + pt];
   sur_features(S, M, (float)k.x, (float)k.y, part, act, p0s, pt, TRAIN ? xs : nullptr, N, p0 + pt);
+  stage_dyn(net, &dsh);
   __syncthreads();
-  sur_hidden<TRAIN>(net, act, red, p0s, phis, xs, zs, N, p0);
+  sur_hidden<TRAIN>(net, &dsh, act, red, p0s, phis, xs, zs, N, p0);
   if (phi && tid < SUR_PTS) phi[p0 + tid] = phis[tid];
   if (!TRAIN) return;
 
@@ -565,6 +579,7 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_plan(PlanDev P, ExecBufs
   __shared__ float red[4 * SUR_PTS * 2];
   __shared__ float2 phis[SUR_PTS];
   __shared__ float2 p0s[SUR_PTS];
+  __shared__ __attribute__((aligned(16))) SurDyn dsh;
   const int tid = threadIdx.x;
   const TileRec tr = tile_rec(P.t_sur, blockIdx.x);
   const int inst = tr.inst;
@@ -578,8 +593,9 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_plan(PlanDev P, ExecBufs
     const bool ok = lane_setup(D, tr.local * SUR_PTS + pt, L);
     sur_features(S, M, ok ? (float)L.lx : 0.0f, ok ? (float)L.ly : 0.0f, part, act, p0s, pt, nullptr, 0, 0);
   }
+  stage_dyn(P.sur_net, &dsh);
   __syncthreads();
-  sur_hidden<false>(P.sur_net, act, red, p0s, phis, nullptr, nullptr, 0, 0);
+  sur_hidden<false>(P.sur_net, &dsh, act, red, p0s, phis, nullptr, nullptr, 0, 0);
   Lane L;
   if (tid < SUR_PTS && lane_setup(D, tr.local * SUR_PTS + pt, L)) {
     if (L.kind == LANE_ORIGIN) {

@@ -119,9 +119,10 @@ def test_forward_and_gradient_match_numpy(metset, net):
     for name, (o, s) in off.items():
         sz = int(np.prod(s))
         assert rel_l2(gk[o : o + sz], grad[o : o + sz]) < 1e-2, name
-    # inference entry point = training forward
+    # the inference entry point computes the same network (fp32 summation
+    # order may differ between instantiations)
     phi2 = net.phi(metset, blk, kxy)
-    assert torch.equal(phi2, phi)
+    assert rel_l2(phi2.cpu().numpy(), phi.cpu().numpy()) < 1e-5
 
 
 def test_adam_matches_numpy(net):
