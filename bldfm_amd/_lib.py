@@ -88,6 +88,14 @@ class SurHull(ctypes.Structure):
     ]
 
 
+class Km01(ctypes.Structure):
+    """Mirror of ``bldfm_km01``."""
+
+    _fields_ = [(f, ctypes.c_double) for f in ("zm", "z0", "ws", "ustar", "mo_len", "sigma_v", "xmin", "xmax",
+                                               "ymin", "ymax", "res", "mx", "my", "wd")] + [
+        ("has_wd", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -121,6 +129,8 @@ SIGNATURES = {
     "bldfm_sur_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I32, _P]),
     "bldfm_plan_set_surrogate": (ctypes.c_int, [_P, _P, ctypes.POINTER(SurHull), _D]),
     "bldfm_plan_sur_stats": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "bldfm_km01_grid": (ctypes.c_int, [ctypes.POINTER(Km01), ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "bldfm_km01_footprint": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Km01), _I32, _P, _P, _P]),
 }
 
 _lock = threading.Lock()

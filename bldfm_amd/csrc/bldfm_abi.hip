@@ -848,4 +848,92 @@ int bldfm_plan_sur_stats(const bldfm_plan* P, const void* workspace, double* err
   return 0;
 }
 
+// ------------------------------------------------------------------ KM01
+// Grid length of numpy's arange(start, stop, step): ceil((stop - start) / step).
+static int64_t arange_len(double start, double stop, double step) {
+  const double n = std::ceil((stop - start) / step);
+  return n > 0 ? (int64_t)n : 0;
+}
+
+int bldfm_km01_grid(const bldfm_km01* p, int32_t* nx, int32_t* ny) {
+  if (!p || !(p->res > 0.0)) return fail(BLDFM_EINVAL, "grid_res must be positive");
+  const int64_t x = arange_len(p->xmin + 0.5 * p->res, p->xmax, p->res);
+  const int64_t y = arange_len(p->ymax - 0.5 * p->res, p->ymin, -p->res);
+  if (x > INT32_MAX || y > INT32_MAX) return fail(BLDFM_ELIMIT, "grid too large");
+  if (nx) *nx = (int32_t)x;
+  if (ny) *ny = (int32_t)y;
+  return 0;
+}
+
+// Scalars of Eqs. (9), (11), (18), (19), (21), (31)-(36) of Kormann & Meixner (2001).
+static void km01_scalars(const bldfm_km01& p, Km01Dev& D) {
+  const double k = 0.4, zm = p.zm, L = p.mo_len;
+  double phim, phic, psim, n;
+  if (L < 0) {
+    const double a = 1.0 - 16.0 * zm / L;
+    phim = pow(a, -0.25);
+    phic = pow(a, -0.5);
+    const double x = pow(a, 0.25);
+    psim = -2.0 * log(0.5 * (1.0 + x)) - log(0.5 * (1.0 + x * x)) + 2.0 * atan(x) - BLDFM_PI * 0.5;
+    n = (1.0 - 24.0 * zm / L) / a;
+  } else {
+    phim = 1.0 + 5.0 * zm / L;
+    phic = phim;
+    psim = 5.0 * zm / L;
+    n = 1.0 / (1.0 + 5.0 * zm / L);
+  }
+  const double m = p.ustar * phim / (k * p.ws);
+  const double kappa = k * zm * p.ustar / (phic * pow(zm, n));
+  const double U = p.ustar * (log(zm / p.z0) + psim) / (k * pow(zm, m));
+  D.valid = U >= 0.0 ? 1 : 0;
+  if (!D.valid) return;
+  const double r = 2.0 + m - n, mu = (1.0 + m) / r, Xi = U * pow(zm, r) / (r * r * kappa);
+  const double mr = m / r;
+  const double A = U / (std::tgamma(1.0 / r) * p.sigma_v) * pow(kappa * r * r / U, mr);
+  const double num = (1.0 / sqrt(2.0 * BLDFM_PI)) * pow(Xi, mu);
+  D.scale = p.res * p.res * num * A;
+  D.e1 = mr - 2.0 - mu;
+  D.e2 = mr - 1.0;
+  D.Xi = Xi;
+  D.gA = std::tgamma(mu) * A;
+}
+
+int bldfm_km01_footprint(int device, const bldfm_km01* p, int32_t n, double* ffm, int32_t* status,
+                         hipStream_t stream) {
+  if (!p || n <= 0 || !ffm) return fail(BLDFM_EINVAL, "bad km01 arguments");
+  std::vector<Km01Dev> D(n);
+  int64_t off = 0, maxc = 0;
+  for (int i = 0; i < n; ++i) {
+    int32_t nx = 0, ny = 0;
+    const int rc = bldfm_km01_grid(p + i, &nx, &ny);
+    if (rc) return rc;
+    Km01Dev& d = D[i];
+    memset(&d, 0, sizeof(d));
+    d.xmin = p[i].xmin; d.ymax = p[i].ymax; d.res = p[i].res; d.mx = p[i].mx; d.my = p[i].my;
+    d.has_wd = p[i].has_wd ? 1 : 0;
+    d.wd_rad = p[i].wd * (BLDFM_PI / 180.0);
+    d.nx = nx; d.ny = ny; d.off = off;
+    km01_scalars(p[i], d);
+    if (status) status[i] = d.valid ? 0 : 1;
+    off += (int64_t)nx * ny;
+    maxc = std::max(maxc, (int64_t)nx * ny);
+  }
+  DeviceGuard guard(device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  void* dd = nullptr;
+  hipError_t e = hipMalloc(&dd, sizeof(Km01Dev) * n);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(km01 records)");
+  e = hipMemcpyAsync(dd, D.data(), sizeof(Km01Dev) * n, hipMemcpyHostToDevice, stream);
+  int rc = 0;
+  if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync(km01 records)");
+  if (!rc) {
+    const int lr = bldfm_launch_km01((const Km01Dev*)dd, n, maxc, ffm, stream);
+    if (lr) rc = launch_fail(lr, "km01");
+  }
+  e = hipStreamSynchronize(stream);  // the records and their host copy must outlive the kernel
+  if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+  (void)hipFree(dd);
+  return rc;
+}
+
 }  // extern "C"

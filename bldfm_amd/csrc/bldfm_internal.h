@@ -72,6 +72,15 @@ int bldfm_launch_transfer(const double* tab, int rows, const int32_t* nzs, const
                           const int32_t* blk_met, int nblk, const double2* kxy, double2* g, int exact,
                           hipStream_t stream);
 int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream);
+// Kormann & Meixner footprint grids: per footprint the cell geometry and the
+// scalars of the closed form, f = scale x^e1 exp(-Xi/x - (gA y x^e2)^2 / 2).
+struct Km01Dev {
+  double xmin, ymax, res, mx, my, wd_rad;
+  double scale, e1, e2, Xi, gA;
+  int64_t off;
+  int32_t nx, ny, has_wd, valid;
+};
+int bldfm_launch_km01(const Km01Dev* P, int n, int64_t max_cells, double* out, hipStream_t stream);
 // F0/F1: device-resident plans of the two axis lengths n0 (columns) and n1 (rows)
 int bldfm_launch_fft2(const FftPlan* F0, int32_t n0, const double2* tw0, const FftPlan* F1, int32_t n1,
                       const double2* tw1, int32_t batch, const double2* in, double2* mid, double2* out,

@@ -397,6 +397,33 @@ __global__ __launch_bounds__(256) void k_sur_gate(PlanDev P, ExecBufs B) {
   if (!(err <= P.sur_tol)) B.mode[i] = 1;
 }
 
+// ---------------------------------------------------------------- KM01
+// Kormann & Meixner (2001) analytic footprint on a regular grid
+// (src/bldfm/ffm_kormann_meixner.py:110-330, Eqs. 9 and 21): one thread per
+// cell, blockIdx.y = footprint.  Cell (j, i) sits at x = xmin + res/2 + i res,
+// y = ymax - res/2 - j res (numpy arange order), is shifted to the receptor
+// and, with a wind direction, rotated into the along/cross-wind frame.
+__global__ __launch_bounds__(256) void k_km01(const Km01Dev* __restrict__ P, double* __restrict__ out) {
+  const Km01Dev& D = P[blockIdx.y];
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= (int64_t)D.nx * D.ny) return;
+  const int j = (int)(c / D.nx), i = (int)(c % D.nx);
+  double x = (D.xmin + 0.5 * D.res + i * D.res) - D.mx;
+  double y = (D.ymax - 0.5 * D.res - j * D.res) - D.my;
+  if (D.has_wd) {
+    const double rho = sqrt(x * x + y * y);
+    const double th = atan2(y, x) + D.wd_rad - 0.5 * BLDFM_PI;
+    x = rho * cos(th);
+    y = rho * sin(th);
+  }
+  double f = 0.0;
+  if (D.valid && x > 0.0) {
+    const double g = D.gA * y * pow(x, D.e2);
+    f = D.scale * pow(x, D.e1) * exp(-D.Xi / x - 0.5 * (g * g));
+  }
+  out[D.off + c] = f;
+}
+
 // ---------------------------------------------------------------- FFT passes
 extern __shared__ double2 smem[];
 
@@ -783,6 +810,14 @@ int bldfm_launch_transfer(const double* tab, int rows, const int32_t* nzs, const
     k_transfer<true><<<grid, 256, 0, stream>>>(tab, rows, nzs, lvls, nmet, blk_met, nblk, kxy, g);
   else
     k_transfer<false><<<grid, 256, 0, stream>>>(tab, rows, nzs, lvls, nmet, blk_met, nblk, kxy, g);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int bldfm_launch_km01(const Km01Dev* P, int n, int64_t max_cells, double* out, hipStream_t stream) {
+  if (n <= 0 || max_cells <= 0) return 0;
+  dim3 grid((unsigned)((max_cells + 255) / 256), (unsigned)n);
+  k_km01<<<grid, 256, 0, stream>>>(P, out);
   LAUNCH_CHECK();
   return 0;
 }

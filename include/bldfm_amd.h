@@ -277,6 +277,29 @@ int bldfm_plan_set_surrogate(bldfm_plan* plan, const void* net, const bldfm_sur_
 int bldfm_plan_sur_stats(const bldfm_plan* plan, const void* workspace, double* err, int32_t* mode,
                          hipStream_t stream);
 
+/* ------------------------------------------------------------------------
+ * Kormann & Meixner (2001) analytic footprint, the reference's comparison
+ * model (src/bldfm/ffm_kormann_meixner.py:110-330).  One record per
+ * footprint; the grid is numpy's arange(xmin + res/2, xmax, res) by
+ * arange(ymax - res/2, ymin, -res) (rows from the top).
+ */
+typedef struct bldfm_km01 {
+  double zm, z0, ws, ustar, mo_len, sigma_v;   /* estimateFootprint arguments */
+  double xmin, xmax, ymin, ymax, res;          /* grid_domain, grid_res */
+  double mx, my;                               /* mxy */
+  double wd;                                   /* wind direction [deg], used when has_wd */
+  int32_t has_wd, reserved;
+} bldfm_km01;
+
+/* Grid shape (rows ny, columns nx) of one record. */
+int bldfm_km01_grid(const bldfm_km01* p, int32_t* nx, int32_t* ny);
+
+/* Footprints of n records into ffm (device, records' ny*nx grids one after
+ * another).  status[i] (host) is 0, or 1 when Eq. (11) gives U < 0 (the
+ * reference then returns an all-zero footprint, as here). */
+int bldfm_km01_footprint(int device, const bldfm_km01* p, int32_t n, double* ffm, int32_t* status,
+                         hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,9 +16,10 @@ reference source text (which is read, never imported or executed):
   (src/bldfm/solver.py:16-374), with numpy's pocketfft standing in for
   FFTW (same DFT definition and ``norm`` conventions,
   src/bldfm/fft_manager.py:78-102).
-* ``oracle.dft``     -- brute-force direct 2-D DFT for tiny grids.
-* ``oracle.km01``    -- Kormann & Meixner (2001) analytic footprint
-  (src/bldfm/ffm_kormann_meixner.py:110-330).
+* ``oracle.km01``    -- Kormann & Meixner (2001) analytic footprint and
+  roughness estimate (src/bldfm/ffm_kormann_meixner.py:16-474).
+* ``oracle.surrogate`` -- the learned surrogate network (no reference
+  counterpart) in numpy FP64, with the kernels' bf16 roundings on request.
 
 How each function is pinned (tests/test_oracle_*.py):
 
@@ -32,13 +33,16 @@ How each function is pinned (tests/test_oracle_*.py):
 * analytic branch: closed form e^{-lambda h} checked mode by mode, and the
   numerical solver agrees with it to 1e-3 of the maximum
   (tests/test_integration.py:28-84).
-* FFT conventions and the truncation/untruncation index maps: brute-force
-  direct DFT on tiny grids.
+* FFT conventions and the truncation/untruncation index maps: a brute-force
+  direct DFT on tiny grids (tests/test_oracle.py).
 * ``psi``: Businger-Dyer known answers (tests/test_ffm_kormann_meixner.py:77-89);
   ``vertical_profiles``: z[0]=z0, z[n]=zm, |U(zm)| reproduced by the MOST
   log law, CONSTANT K = kappa u* zm / prsc (tests/test_pbl_model.py:49-53).
 * ``compute_wind_fields``: closed form for the four cardinal directions.
-* KM01: known answers of tests/test_ffm_kormann_meixner.py.
+* KM01: known answers of tests/test_ffm_kormann_meixner.py, the closed-form
+  position of the along-wind maximum, mass conservation on a large domain.
+* surrogate network: its hand-written backward pass against central
+  differences of its loss (FP64), bf16 rounding against torch's.
 
 Parity unpinned: none of the above pins the FP64 (``precision="double"``)
 solver end-to-end at better than the complex64 rounding the fixtures went

@@ -103,3 +103,18 @@ def test_cache_roundtrip(tmp_path):
     other[4] = (10.0, 10.0)
     assert c.get(*other) is None
     assert c.clear() == 1 and c.get(*args) is None
+
+
+def test_cli_dry_run(tmp_path, caplog):
+    import logging
+
+    from bldfm_amd.cli import main
+
+    p = tmp_path / "c.yaml"
+    p.write_text("domain: {nx: 64, ny: 32, xmax: 200.0, ymax: 100.0, nz: 8, ref_lat: 50.95, ref_lon: 11.586}\n"
+                 "towers: [{name: T, lat: 50.9503, lon: 11.5865, z_m: 5.0}]\n"
+                 "met: {ustar: 0.4, mol: -50.0}\n")
+    with caplog.at_level(logging.INFO, logger="bldfm_amd.cli"):
+        assert main(["run", str(p), "--dry-run"]) == 0
+    text = caplog.text
+    assert "Dry run complete" in text and "Tower 'T'" in text and "Timesteps: 1" in text

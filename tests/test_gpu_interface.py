@@ -88,3 +88,25 @@ def test_cache_hit_returns_stored_result(tmp_path):
     assert len(list((tmp_path / "cache").glob("*.npz"))) == 1
     r2 = itf.run_bldfm_single(cfg, cfg.towers[0], cache=cache)
     assert np.array_equal(r1["flx"], r2["flx"]) and np.array_equal(r1["conc"], r2["conc"])
+
+
+def test_cli_run_writes_results(tmp_path):
+    from bldfm_amd import interface as itf
+    from bldfm_amd.cli import main
+
+    p = tmp_path / "c.yaml"
+    p.write_text("domain: {nx: 64, ny: 48, xmax: 400.0, ymax: 300.0, nz: 12, modes: [64, 48], "
+                 "ref_lat: 50.95, ref_lon: 11.586}\n"
+                 "towers: [{name: A, lat: 50.9502, lon: 11.5885, z_m: 6.0}]\n"
+                 "met: {ustar: [0.3, 0.45], mol: [-50.0, 200.0], wind_speed: [4.0, 6.0], wind_dir: [270.0, 200.0]}\n"
+                 "solver: {footprint: true, precision: double}\n")
+    out = tmp_path / "r.npz"
+    assert main(["run", str(p), "--out", str(out)]) == 0
+    from bldfm_amd.config_parser import load_config
+
+    cfg = load_config(p)
+    ref = itf.run_bldfm_timeseries(cfg, cfg.towers[0])
+    with np.load(out) as d:
+        for i in range(2):
+            assert np.array_equal(d[f"A/{i}/flx"], ref[i]["flx"])
+            assert np.array_equal(d[f"A/{i}/X"], ref[i]["grid"][0])

@@ -1,0 +1,83 @@
+"""Kormann & Meixner footprints on the GPU against the oracle, batched and
+single, and the reference's BLDFM-vs-KM01 neutral comparison
+(tests/test_integration.py:534-660) at the same reduced size."""
+
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import km01
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    dict(zm=10.0, z0=0.1, ws=4.0, ustar=0.4, mo_len=-100.0, sigma_v=0.3, grid_domain=[-200, 200, -100, 100],
+         grid_res=5.0, mxy=[0, 0], wd=270.0),
+    dict(zm=10.0, z0=0.1, ws=4.0, ustar=0.4, mo_len=1e6, sigma_v=0.3, grid_domain=[-500, 3000, -1000, 1000],
+         grid_res=5.0, mxy=[0, 0], wd=270.0),
+    dict(zm=25.0, z0=0.3, ws=6.0, ustar=0.5, mo_len=80.0, sigma_v=0.5, grid_domain=[0, 1000, 0, 800],
+         grid_res=4.0, mxy=[300.0, 500.0], wd=33.0),
+    dict(zm=8.0, z0=0.05, ws=3.0, ustar=0.3, mo_len=-30.0, sigma_v=0.4, grid_domain=[-100, 900, -300, 300],
+         grid_res=2.5, mxy=[0, 0], wd=None),
+]
+
+
+def _oracle(c):
+    c = dict(c)
+    c["L"] = c.pop("mo_len")
+    return km01.estimate_footprint(**c)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_footprint_matches_oracle(case):
+    from bldfm_amd.ffm_kormann_meixner import estimateFootprint
+
+    gx, gy, f = estimateFootprint(**CASES[case])
+    ox, oy, of = _oracle(CASES[case])
+    assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
+    assert rel_l2(f, of) < 1e-12 and np.all(f >= 0)
+
+
+def test_batched_equals_single_and_warns():
+    from bldfm_amd.ffm_kormann_meixner import estimate_footprints, estimateFootprint
+
+    bad = dict(CASES[0], z0=100.0)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        res = estimate_footprints(CASES + [bad])
+    assert len(w) == 1 and np.all(res[-1][2] == 0.0)
+    for c, (gx, gy, f) in zip(CASES, res):
+        assert np.array_equal(f, estimateFootprint(**c)[2])
+
+
+def test_bldfm_and_km01_neutral_comparison():
+    """Both models on the reference's reduced neutral case: KM01 >= 0, BLDFM
+    >= -1e-4 (Gibbs ringing at the receptor), integrals within [0.3, 1.2]
+    and within a factor 3 of each other."""
+    from bldfm_amd.ffm_kormann_meixner import estimateFootprint
+    from bldfm_amd.pbl_model import vertical_profiles
+    from bldfm_amd.solver import steady_state_transport_solver
+    from bldfm_amd.utils import ideal_source
+
+    nxy, domain, modes, nz, halo = (64, 256), (50.0, 200.0), (64, 256), 32, 300.0
+    meas_pt, zm, wind, ustar, z0, mol, sigma_v = (25.0, 20.0), 10.0, (0.0, -5.0), 0.668, 0.5, 1e9, 0.32
+    dx, dy = domain[0] / nxy[0], domain[1] / nxy[1]
+    z, profs = vertical_profiles(nz, zm, wind, z0=z0, mol=mol, closure="MOST")
+    _, _, flx = steady_state_transport_solver(ideal_source(nxy, domain), z, profs, domain, nz, modes=modes,
+                                              meas_pt=meas_pt, footprint=True, halo=halo)
+    wd = np.arctan(wind[0] / wind[1]) * 180.0 / np.pi
+    _, _, ffm = estimateFootprint(zm=zm, z0=z0, ws=np.hypot(*wind), ustar=ustar, mo_len=mol, sigma_v=sigma_v,
+                                  grid_domain=[0, domain[0], 0, domain[1]], grid_res=dx, mxy=meas_pt, wd=wd)
+    assert np.all(ffm >= 0) and np.all(flx >= -1e-4)
+    ib = float(np.sum(flx) * dx * dy)
+    ik = float(np.sum(ffm))
+    assert 0.3 < ik < 1.2 and 0.3 < ib < 1.2 and 0.3 < ib / ik < 3.0
+    # peaks upwind of the receptor (wind blows toward -y) and within 50 m
+    X, Y = np.meshgrid(np.linspace(0, domain[0], nxy[0], endpoint=False),
+                       np.linspace(0, domain[1], nxy[1], endpoint=False))
+    kx, ky = np.meshgrid(np.arange(0.5 * dx, domain[0], dx), np.arange(domain[1] - 0.5 * dx, 0, -dx))
+    ib_, ik_ = np.unravel_index(np.argmax(flx), flx.shape), np.unravel_index(np.argmax(ffm), ffm.shape)
+    assert Y[ib_] > meas_pt[1] and ky[ik_] > meas_pt[1]
+    assert np.hypot(X[ib_] - kx[ik_], Y[ib_] - ky[ik_]) < 50.0
