@@ -1,12 +1,15 @@
 // LDS-resident mixed-radix Stockham FFT (FP64 complex) for one workgroup.
 //
-// The transform of length n lives in an LDS buffer of n double2.  Each stage
-// of radix R with current sub-transform length Ns reads butterfly inputs
-// buf[j + r*n/R] (j in [0, n/R)), applies the twiddles W_n^{r*k*n/(Ns*R)}
-// with k = j mod Ns, performs a length-R DFT and writes the outputs to
-// buf[(j - k)*R + k + r*Ns] (self-sorting, natural order at the end).  The
-// stage is done in place: every thread first gathers its butterflies into
-// registers, then the workgroup synchronises, then all threads scatter.
+// Each stage of radix R with current sub-transform length Ns reads butterfly
+// inputs x[j + r*n/R] (j in [0, n/R)), applies the twiddles
+// W_n^{r*k*n/(Ns*R)} with k = j mod Ns, performs a length-R DFT and writes the
+// outputs to x[(j - k)*R + k + r*Ns] (self-sorting, natural order at the
+// end).  The first stage gathers from the caller's source (global memory:
+// band scatter, zero padding and strides folded into the index map) and the
+// last stage scatters to the caller's sink (cropping, scaling, output
+// format); the stages between live in an LDS buffer of n double2, in place:
+// every thread first gathers its butterflies into registers, then the
+// workgroup synchronises, then all threads scatter.
 //
 // W_n^t = e^{-2 pi i t/n} comes from a precomputed table; the e^{+i} transform
 // conjugates it.  A thread handles at most MAXE complex values per stage, so
@@ -105,9 +108,40 @@ __device__ __forceinline__ void butterfly<8>(double2* x, bool inv) {
 __device__ __forceinline__ uint64_t div_magic(int d) { return (1ull << 32) / (uint64_t)d + 1ull; }
 __device__ __forceinline__ int fdiv(int j, uint64_t m) { return (int)(((uint64_t)(uint32_t)j * m) >> 32); }
 
-template <int R, int MAXB>
-__device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const double2* __restrict__ tw,
-                                        bool inv, int tid, int T) {
+// Element i of an LDS transform buffer lives at i ^ ((i >> 3) & 15) when the
+// length is a multiple of 16 (a bijection within every aligned block of 16),
+// else at i.  The first Stockham stages write with element strides R*Ns
+// (8, 16, 64, ...): unswizzled, 8 or 16 lanes of one ds_write_b128 would hit
+// the same banks; swizzled, every aligned group of 8 and of 16 lanes hits
+// distinct banks in every stage of the factorisations used here.
+struct Lds {
+  double2* p;
+  int swz;
+  __device__ __forceinline__ double2& operator[](int i) const { return p[i ^ ((i >> 3) & swz)]; }
+};
+__device__ __forceinline__ Lds lds_view(double2* p, int n) { return Lds{p, (n & 15) ? 0 : 15}; }
+
+// Source and sink of the intermediate stages: the LDS buffer.
+struct LdsIn {
+  Lds b;
+  __device__ __forceinline__ double2 operator()(int i) const { return b[i]; }
+};
+struct LdsOut {
+  Lds b;
+  __device__ __forceinline__ void operator()(int i, double2 v) const { b[i] = v; }
+};
+
+template <class A, class B> struct same_t { static constexpr bool value = false; };
+template <class A> struct same_t<A, A> { static constexpr bool value = true; };
+
+// One radix-R stage: gather the butterfly inputs in(j + r*nb), twiddle,
+// length-R DFT, scatter to out(base + r*Ns).  When the sink is the LDS buffer
+// the workgroup synchronises before the scatter (the stage works in place)
+// and after it (the next stage reads it).
+template <int R, int MAXB, bool INV, class In, class Out>
+__device__ __forceinline__ void stage_r(In in, Out out, int n, int Ns, const double2* __restrict__ tw,
+                                        int tid, int T) {
+  constexpr bool to_lds = same_t<Out, LdsOut>::value;
   const int nb = n / R;
   const int stride = n / (Ns * R);
   const uint64_t mNs = div_magic(Ns);
@@ -118,12 +152,12 @@ __device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const doubl
     if (j < nb) {
       const int k = j - fdiv(j, mNs) * Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[b][r] = buf[j + r * nb];
+      for (int r = 0; r < R; ++r) x[b][r] = in(j + r * nb);
       if (k != 0) {
         // W^{r k stride} for r = 1..R-1 from one table entry by repeated
         // multiplication (error grows like r*eps, not amplified downstream)
         double2 w1 = tw[k * stride];
-        if (inv) w1.y = -w1.y;
+        if (INV) w1.y = -w1.y;
         double2 w = w1;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -131,10 +165,10 @@ __device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const doubl
           if (r + 1 < R) w = cmul(w, w1);
         }
       }
-      butterfly<R>(x[b], inv);
+      butterfly<R>(x[b], INV);
     }
   }
-  __syncthreads();
+  if (to_lds) __syncthreads();
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int j = tid + b * T;
@@ -143,17 +177,17 @@ __device__ __forceinline__ void stage_r(double2* buf, int n, int Ns, const doubl
       const int k = j - q * Ns;
       const int base = q * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[base + r * Ns] = x[b][r];
+      for (int r = 0; r < R; ++r) out(base + r * Ns, x[b][r]);
     }
   }
-  __syncthreads();
+  if (to_lds) __syncthreads();
 }
 
 // Any radix: one output per (thread, slot), out[o] = sum_m x_m W_n^{m*stride*(k + r*Ns)}.
-template <int MAXE>
-__device__ __forceinline__ void stage_generic(double2* buf, int n, int R, int Ns,
-                                              const double2* __restrict__ tw, bool inv, int tid,
-                                              int T) {
+template <int MAXE, bool INV, class In, class Out>
+__device__ __forceinline__ void stage_generic(In in, Out out, int n, int R, int Ns,
+                                              const double2* __restrict__ tw, int tid, int T) {
+  constexpr bool to_lds = same_t<Out, LdsOut>::value;
   const int nb = n / R;
   const int stride = n / (Ns * R);
   const uint64_t mNs = div_magic(Ns), mR = div_magic(R);
@@ -172,44 +206,78 @@ __device__ __forceinline__ void stage_generic(double2* buf, int n, int R, int Ns
       int t = 0;
       for (int m = 0; m < R; ++m) {
         double2 w = tw[t];
-        if (inv) w.y = -w.y;
-        acc = cadd(acc, cmul(buf[j + m * nb], w));
+        if (INV) w.y = -w.y;
+        acc = cadd(acc, cmul(in(j + m * nb), w));
         t += step;
         if (t >= n) t -= n;
       }
       y[e] = acc;
     }
   }
-  __syncthreads();
+  if (to_lds) __syncthreads();
 #pragma unroll
   for (int e = 0; e < MAXE; ++e) {
     const int o = tid + e * T;
-    if (o < n) buf[o] = y[e];
+    if (o < n) out(o, y[e]);
   }
-  __syncthreads();
+  if (to_lds) __syncthreads();
 }
 
-// Full transform on buf (n points) with the plan at Pg (device memory,
-// read through the constant address space: the plan is uniform);
-// inv selects e^{+i}.  A thread handles at most MAXE values per stage: the
-// launch uses T >= ceil(n/MAXE).
-template <int MAXE>
-__device__ __forceinline__ void lds_fft(double2* buf, const FftPlan* Pg, const double2* __restrict__ tw,
-                                        bool inv, int tid, int T) {
+template <int MAXE, bool INV, class In, class Out>
+__device__ __forceinline__ void run_stage(int R, In in, Out out, int n, int Ns, const double2* __restrict__ tw,
+                                          int tid, int T) {
+  switch (R) {
+    case 2: stage_r<2, (MAXE + 1) / 2, INV>(in, out, n, Ns, tw, tid, T); break;
+    case 3: stage_r<3, (MAXE + 2) / 3, INV>(in, out, n, Ns, tw, tid, T); break;
+    case 4: stage_r<4, (MAXE + 3) / 4, INV>(in, out, n, Ns, tw, tid, T); break;
+    case 5: stage_r<5, (MAXE + 4) / 5, INV>(in, out, n, Ns, tw, tid, T); break;
+    case 8: stage_r<8, (MAXE + 7) / 8, INV>(in, out, n, Ns, tw, tid, T); break;
+    default: stage_generic<MAXE, INV>(in, out, n, R, Ns, tw, tid, T); break;
+  }
+}
+
+// Full transform of the n values in(0..n-1) into out(0..n-1), with the plan
+// at Pg (device memory, read through the constant address space: the plan is
+// uniform); INV selects e^{+i}.  The first stage gathers straight from `in`
+// and the last scatters straight to `out`, so a transform whose source and
+// sink are global memory makes n-point LDS round trips only between stages;
+// buf (n double2) holds the intermediate stages.  A thread handles at most
+// MAXE values per stage: the launch uses T >= ceil(n/MAXE).  On return every
+// LDS read of the transform has completed in every thread only if `out` is
+// the LDS buffer (then the workgroup has synchronised); a caller that reuses
+// buf for its next transform with a global source needs no barrier: the
+// first stage synchronises before its scatter.
+template <int MAXE, bool INV, class In, class Out>
+__device__ __forceinline__ void lds_fft(Lds buf, const FftPlan* Pg, const double2* __restrict__ tw, int tid0, int T0,
+                                        In in, Out out) {
+  constexpr bool in_lds = same_t<In, LdsIn>::value, out_lds = same_t<Out, LdsOut>::value;
   typedef const __attribute__((address_space(4))) int32_t cint32;
   const cint32* pc = (const cint32*)Pg;
   const int n = pc[0];
   const int nstage = pc[1];
+  if (nstage == 0) {  // n == 1
+    if (tid0 == 0) out(0, in(0));
+    if (out_lds) __syncthreads();
+    return;
+  }
+  const LdsIn li{buf};
+  const LdsOut lo{buf};
   int Ns = 1;
   for (int s = 0; s < nstage; ++s) {
     const int R = pc[2 + s];
-    switch (R) {
-      case 2: stage_r<2, (MAXE + 1) / 2>(buf, n, Ns, tw, inv, tid, T); break;
-      case 3: stage_r<3, (MAXE + 2) / 3>(buf, n, Ns, tw, inv, tid, T); break;
-      case 4: stage_r<4, (MAXE + 3) / 4>(buf, n, Ns, tw, inv, tid, T); break;
-      case 5: stage_r<5, (MAXE + 4) / 5>(buf, n, Ns, tw, inv, tid, T); break;
-      case 8: stage_r<8, (MAXE + 7) / 8>(buf, n, Ns, tw, inv, tid, T); break;
-      default: stage_generic<MAXE>(buf, n, R, Ns, tw, inv, tid, T); break;
+    const bool first = s == 0, last = s == nstage - 1;
+    // Opaque per-stage copies of tid and T: without them the compiler hoists
+    // every stage variant's index arithmetic out of the loop and keeps it
+    // live across all of them (doubling the VGPR count).
+    int tid = tid0, T = T0;
+    asm volatile("" : "+v"(tid), "+s"(T));
+    if (!in_lds && first) {
+      if (!out_lds && last) run_stage<MAXE, INV>(R, in, out, n, Ns, tw, tid, T);
+      else run_stage<MAXE, INV>(R, in, lo, n, Ns, tw, tid, T);
+    } else if (!out_lds && last) {
+      run_stage<MAXE, INV>(R, li, out, n, Ns, tw, tid, T);
+    } else {
+      run_stage<MAXE, INV>(R, li, lo, n, Ns, tw, tid, T);
     }
     Ns *= R;
   }

@@ -427,25 +427,86 @@ __global__ __launch_bounds__(256) void k_km01(const Km01Dev* __restrict__ P, dou
 // ---------------------------------------------------------------- FFT passes
 extern __shared__ double2 smem[];
 
-__device__ __forceinline__ void lds_zero(int n, int tid, int T) {
-  for (int i = tid; i < n; i += T) smem[i] = cmk(0.0, 0.0);
-  __syncthreads();
-}
+// Sources (first stage) and sinks (last stage) of the transforms.  A band of
+// nb compact values (truncated order: f >= 0 first, then f < 0) occupies
+// padded indices [0, lo) and [n - (nb - lo), n) with lo = (nb + 1)/2; the
+// gap between the two halves is zero.
+struct Band {
+  int lo, hi_start, nb;
+  __device__ __forceinline__ Band(int nb_, int n) : lo((nb_ + 1) / 2), hi_start(n - (nb_ - (nb_ + 1) / 2)), nb(nb_) {}
+  // compact index of padded index i, or -1 in the gap
+  __device__ __forceinline__ int compact(int i) const { return i < lo ? i : (i >= hi_start ? i - hi_start + lo : -1); }
+};
 
-// Padded transform input from a compact band of nb values (element stride
-// `stride` in global memory): band values at their padded indices, zeros in
-// the gap between the two halves of the band, one pass, one barrier.
-__device__ __forceinline__ void band_load(double2* buf, const double2* __restrict__ src, int64_t stride,
-                                          int nb, int n, int tid, int T) {
-  const int lo = (nb + 1) / 2;  // compact indices [0, lo) hold f >= 0 at padded [0, lo)
-  const int hi_start = n - (nb - lo);
-  for (int i = tid; i < n; i += T) {
-    double2 v = cmk(0.0, 0.0);
-    if (i < lo) v = src[(int64_t)i * stride];
-    else if (i >= hi_start) v = src[(int64_t)(i - hi_start + lo) * stride];
-    buf[i] = v;
+// Compact band at element stride `stride` in global memory.
+struct BandIn {
+  const double2* __restrict__ src;
+  int64_t stride;
+  Band band;
+  __device__ __forceinline__ double2 operator()(int i) const {
+    const int c = band.compact(i);
+    return c < 0 ? cmk(0.0, 0.0) : src[(int64_t)c * stride];
   }
-  __syncthreads();
+};
+
+// Band of the spectrum, scaled, to a compact array.
+struct BandOut {
+  double2* __restrict__ dst;
+  double fct;
+  Band band;
+  __device__ __forceinline__ void operator()(int o, double2 v) const {
+    const int c = band.compact(o);
+    if (c >= 0) dst[c] = cscale(fct, v);
+  }
+};
+
+// cnt values (element stride `stride`) placed at [off, off + cnt), zeros
+// elsewhere; real input when `re` is set.
+template <class T_>
+struct PadIn {
+  const T_* __restrict__ src;
+  int64_t stride;
+  int off, cnt;
+  __device__ __forceinline__ double2 operator()(int i) const {
+    const unsigned t = (unsigned)(i - off);
+    if (t >= (unsigned)cnt) return cmk(0.0, 0.0);
+    if constexpr (sizeof(T_) == sizeof(double)) return cmk(src[(int64_t)t * stride], 0.0);
+    else return src[(int64_t)t * stride];
+  }
+};
+
+// Outputs [off, off + cnt) to a contiguous array.
+struct CropOut {
+  double2* __restrict__ dst;
+  int off, cnt;
+  __device__ __forceinline__ void operator()(int o, double2 v) const {
+    const unsigned t = (unsigned)(o - off);
+    if (t < (unsigned)cnt) dst[t] = v;
+  }
+};
+
+// Outputs [off, off + cnt) split into conc = cm * Re and flx = Im.
+template <class T_>
+struct SplitOut {
+  T_* __restrict__ conc;
+  T_* __restrict__ flx;
+  double cm;
+  int off, cnt;
+  __device__ __forceinline__ void operator()(int o, double2 v) const {
+    const unsigned t = (unsigned)(o - off);
+    if (t < (unsigned)cnt) {
+      conc[t] = (T_)(cm * v.x);
+      flx[t] = (T_)v.y;
+    }
+  }
+};
+
+template <int MAXE, class In, class Out>
+__device__ __forceinline__ void fft_dir(bool inv, const cplan_t& F, const double2* __restrict__ tw, int tid, int T,
+                                        In in, Out out) {
+  const bldfm_fft::Lds buf = bldfm_fft::lds_view(smem, F.n);
+  if (inv) bldfm_fft::lds_fft<MAXE, true>(buf, (const FftPlan*)&F, tw + F.tw_off, tid, T, in, out);
+  else bldfm_fft::lds_fft<MAXE, false>(buf, (const FftPlan*)&F, tw + F.tw_off, tid, T, in, out);
 }
 
 // Forward, rows: one surface-flux row y placed at [px, px+nx) of nxe points,
@@ -460,16 +521,9 @@ __global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     const int y = tr.local;
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
-    lds_zero(F.n, tid, T);
-    const double* row = B.src + D.off_src + (int64_t)y * D.nx;
-    for (int x = tid; x < D.nx; x += T) smem[D.px + x] = cmk(row[x], 0.0);
-    __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, false, tid, T);
-    const double fct = 1.0 / (double)F.n;
-    double2* aw = B.ws + D.off_aw + (int64_t)y * D.nlx;
-    for (int j = tid; j < D.nlx; j += T)
-      aw[j] = cscale(fct, smem[padded_of_freq(freq_of_trunc(j, D.nlx), F.n)]);
-    __syncthreads();
+    const PadIn<double> in{B.src + D.off_src + (int64_t)y * D.nx, 1, D.px, D.nx};
+    const BandOut out{B.ws + D.off_aw + (int64_t)y * D.nlx, 1.0 / (double)F.n, Band(D.nlx, F.n)};
+    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out);
   }
 }
 
@@ -484,16 +538,9 @@ __global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     const int jx = tr.local;
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
-    lds_zero(F.n, tid, T);
-    const double2* aw = B.ws + D.off_aw;
-    for (int y = tid; y < D.ny; y += T) smem[D.py + y] = aw[(int64_t)y * D.nlx + jx];
-    __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, false, tid, T);
-    const double fct = 1.0 / (double)F.n;
-    double2* q0 = B.ws + D.off_q0 + (int64_t)jx * D.nly;
-    for (int j = tid; j < D.nly; j += T)
-      q0[j] = cscale(fct, smem[padded_of_freq(freq_of_trunc(j, D.nly), F.n)]);
-    __syncthreads();
+    const PadIn<double2> in{B.ws + D.off_aw + jx, D.nlx, D.py, D.ny};
+    const BandOut out{B.ws + D.off_q0 + (int64_t)jx * D.nly, 1.0 / (double)F.n, Band(D.nly, F.n)};
+    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out);
   }
 }
 
@@ -509,11 +556,9 @@ __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     const int64_t local = tr.local;  // s*nxb + cx
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
-    band_load(smem, B.ws + D.off_z + local * D.nyb, 1, D.nyb, F.n, tid, T);
-    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, !D.footprint, tid, T);
-    double2* v = B.ws + D.off_v + local * D.ny;
-    for (int y = tid; y < D.ny; y += T) v[y] = smem[D.py + y];
-    __syncthreads();
+    const BandIn in{B.ws + D.off_z + local * D.nyb, 1, Band(D.nyb, F.n)};
+    const CropOut out{B.ws + D.off_v + local * D.ny, D.py, D.ny};
+    fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in, out);
   }
 }
 
@@ -529,29 +574,16 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
     const int64_t local = tr.local;  // s*ny + y
     const int s = (int)(local / D.ny), y = (int)(local % D.ny);
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
-    band_load(smem, B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, D.nxb, F.n, tid, T);
-    bldfm_fft::lds_fft<MAXE>(smem, (const FftPlan*)&F, P.tw + F.tw_off, !D.footprint, tid, T);
+    const BandIn in{B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, Band(D.nxb, F.n)};
     const int64_t o = D.off_out + local * D.nx;
     // instances served by the surrogate carry no concentration spectrum
     const double cm = (B.mode && B.mode[tr.inst] == 0) ? 0.0 : 1.0;
-    if (D.out_f32) {
-      float* conc = (float*)B.conc + o;
-      float* flx = (float*)B.flx + o;
-      for (int x = tid; x < D.nx; x += T) {
-        const double2 w = smem[D.px + x];
-        conc[x] = (float)(cm * w.x);
-        flx[x] = (float)w.y;
-      }
-    } else {
-      double* conc = (double*)B.conc + o;
-      double* flx = (double*)B.flx + o;
-      for (int x = tid; x < D.nx; x += T) {
-        const double2 w = smem[D.px + x];
-        conc[x] = cm * w.x;
-        flx[x] = w.y;
-      }
-    }
-    __syncthreads();
+    if (D.out_f32)
+      fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in,
+                    SplitOut<float>{(float*)B.conc + o, (float*)B.flx + o, cm, D.px, D.nx});
+    else
+      fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in,
+                    SplitOut<double>{(double*)B.conc + o, (double*)B.flx + o, cm, D.px, D.nx});
   }
 }
 
@@ -659,20 +691,33 @@ int bldfm_launch_profiles(const ProfSetup& S, double* out, hipStream_t stream) {
 // numpy fft2/ifft2 on `batch` contiguous (n0, n1) complex arrays: x-pass
 // over the rows into the workspace, then y-pass over the columns, output
 // multiplied by `scale`.
+struct StridedIn {
+  const double2* __restrict__ src;
+  int64_t stride;
+  __device__ __forceinline__ double2 operator()(int i) const { return src[(int64_t)i * stride]; }
+};
+struct StridedOut {
+  double2* __restrict__ dst;
+  int64_t stride;
+  double scale;
+  __device__ __forceinline__ void operator()(int o, double2 v) const { dst[(int64_t)o * stride] = cscale(scale, v); }
+};
+
+template <int MAXE>
+__device__ __forceinline__ void fft_plain(bool inv, const FftPlan* F, int n, const double2* __restrict__ tw, int tid,
+                                          int T, StridedIn in, StridedOut out) {
+  const bldfm_fft::Lds buf = bldfm_fft::lds_view(smem, n);
+  if (inv) bldfm_fft::lds_fft<MAXE, true>(buf, F, tw, tid, T, in, out);
+  else bldfm_fft::lds_fft<MAXE, false>(buf, F, tw, tid, T, in, out);
+}
+
 template <int MAXE, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_fft2_rows(const FftPlan* F, const double2* __restrict__ tw,
                                                   const double2* __restrict__ in, double2* __restrict__ mid,
                                                   int32_t n, int64_t nrows, bool inv) {
   const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
-    const double2* src = in + r * n;
-    for (int i = tid; i < n; i += T) smem[i] = src[i];
-    __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, F, tw, inv, tid, T);
-    double2* dst = mid + r * n;
-    for (int i = tid; i < n; i += T) dst[i] = smem[i];
-    __syncthreads();
-  }
+  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x)
+    fft_plain<MAXE>(inv, F, n, tw, tid, T, StridedIn{in + r * n, 1}, StridedOut{mid + r * n, 1, 1.0});
 }
 
 template <int MAXE, int MAXT>
@@ -682,13 +727,8 @@ __global__ __launch_bounds__(MAXT) void k_fft2_cols(const FftPlan* F, const doub
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t c = blockIdx.x; c < ncols; c += gridDim.x) {
     const int64_t b = c / n1, col = c % n1;
-    const double2* src = mid + b * (int64_t)n0 * n1 + col;
-    for (int i = tid; i < n0; i += T) smem[i] = src[(int64_t)i * n1];
-    __syncthreads();
-    bldfm_fft::lds_fft<MAXE>(smem, F, tw, inv, tid, T);
-    double2* dst = out + b * (int64_t)n0 * n1 + col;
-    for (int i = tid; i < n0; i += T) dst[(int64_t)i * n1] = cscale(scale, smem[i]);
-    __syncthreads();
+    const int64_t o = b * (int64_t)n0 * n1 + col;
+    fft_plain<MAXE>(inv, F, n0, tw, tid, T, StridedIn{mid + o, n1}, StridedOut{out + o, n1, scale});
   }
 }
 
