@@ -65,6 +65,12 @@ bool factorize(int n, FftPlan* P) {
     while (m % p == 0) { if (!push(p)) return false; m /= p; }
     if (p * p > m && m > 1) { if (!push(m)) return false; m = 1; }
   }
+  int64_t ns = 1;
+  for (int s = 0; s < P->nstage; ++s) {
+    P->stride[s] = (int32_t)(n / (ns * P->radix[s]));
+    P->magic[s] = ns >= 2 ? (uint32_t)((1ull << 32) / (uint64_t)ns + 1ull) : 0u;
+    ns *= P->radix[s];
+  }
   return true;
 }
 

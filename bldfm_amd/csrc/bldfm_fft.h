@@ -134,35 +134,43 @@ struct LdsOut {
 template <class A, class B> struct same_t { static constexpr bool value = false; };
 template <class A> struct same_t<A, A> { static constexpr bool value = true; };
 
+// Stage constants: radix R, Ns, twiddle index step n/(Ns*R), and the
+// multiply-high divisor of Ns (unused in the first stage, where Ns = 1).
+struct StageK {
+  int Ns, stride;
+  uint32_t magic;
+};
+
 // One radix-R stage: gather the butterfly inputs in(j + r*nb), twiddle,
 // length-R DFT, scatter to out(base + r*Ns).  When the sink is the LDS buffer
 // the workgroup synchronises before the scatter (the stage works in place)
-// and after it (the next stage reads it).
-template <int R, int MAXB, bool INV, class In, class Out>
-__device__ __forceinline__ void stage_r(In in, Out out, int n, int Ns, const double2* __restrict__ tw,
+// and after it (the next stage reads it).  FIRST: Ns = 1 (no twiddles,
+// k = 0, q = j).
+template <int R, int MAXB, bool INV, bool FIRST, class In, class Out>
+__device__ __forceinline__ void stage_r(In in, Out out, int n, StageK K, const double2* __restrict__ tw,
                                         int tid, int T) {
   constexpr bool to_lds = same_t<Out, LdsOut>::value;
   const int nb = n / R;
-  const int stride = n / (Ns * R);
-  const uint64_t mNs = div_magic(Ns);
   double2 x[MAXB][R];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int j = tid + b * T;
     if (j < nb) {
-      const int k = j - fdiv(j, mNs) * Ns;
 #pragma unroll
       for (int r = 0; r < R; ++r) x[b][r] = in(j + r * nb);
-      if (k != 0) {
-        // W^{r k stride} for r = 1..R-1 from one table entry by repeated
-        // multiplication (error grows like r*eps, not amplified downstream)
-        double2 w1 = tw[k * stride];
-        if (INV) w1.y = -w1.y;
-        double2 w = w1;
+      if (!FIRST) {
+        const int k = j - (int)__umulhi((unsigned)j, K.magic) * K.Ns;
+        if (k != 0) {
+          // W^{r k stride} for r = 1..R-1 from one table entry by repeated
+          // multiplication (error grows like r*eps, not amplified downstream)
+          double2 w1 = tw[k * K.stride];
+          if (INV) w1.y = -w1.y;
+          double2 w = w1;
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          x[b][r] = cmul(x[b][r], w);
-          if (r + 1 < R) w = cmul(w, w1);
+          for (int r = 1; r < R; ++r) {
+            x[b][r] = cmul(x[b][r], w);
+            if (r + 1 < R) w = cmul(w, w1);
+          }
         }
       }
       butterfly<R>(x[b], INV);
@@ -173,35 +181,39 @@ __device__ __forceinline__ void stage_r(In in, Out out, int n, int Ns, const dou
   for (int b = 0; b < MAXB; ++b) {
     const int j = tid + b * T;
     if (j < nb) {
-      const int q = fdiv(j, mNs);
-      const int k = j - q * Ns;
-      const int base = q * Ns * R + k;
+      int base;
+      if (FIRST) {
+        base = j * R;
+      } else {
+        const int q = (int)__umulhi((unsigned)j, K.magic);
+        base = q * K.Ns * (R - 1) + j;  // q*Ns*R + k with k = j - q*Ns
+      }
 #pragma unroll
-      for (int r = 0; r < R; ++r) out(base + r * Ns, x[b][r]);
+      for (int r = 0; r < R; ++r) out(base + r * (FIRST ? 1 : K.Ns), x[b][r]);
     }
   }
   if (to_lds) __syncthreads();
 }
 
 // Any radix: one output per (thread, slot), out[o] = sum_m x_m W_n^{m*stride*(k + r*Ns)}.
-template <int MAXE, bool INV, class In, class Out>
-__device__ __forceinline__ void stage_generic(In in, Out out, int n, int R, int Ns,
+template <int MAXE, bool INV, bool FIRST, class In, class Out>
+__device__ __forceinline__ void stage_generic(In in, Out out, int n, int R, StageK K,
                                               const double2* __restrict__ tw, int tid, int T) {
   constexpr bool to_lds = same_t<Out, LdsOut>::value;
   const int nb = n / R;
-  const int stride = n / (Ns * R);
-  const uint64_t mNs = div_magic(Ns), mR = div_magic(R);
+  const int Ns = FIRST ? 1 : K.Ns;
+  const uint64_t mR = div_magic(R);
   double2 y[MAXE];
 #pragma unroll
   for (int e = 0; e < MAXE; ++e) {
     const int o = tid + e * T;
     if (o < n) {
-      const int q = fdiv(o, mNs);      // o / Ns
+      const int q = FIRST ? o : (int)__umulhi((unsigned)o, K.magic);  // o / Ns
       const int k = o - q * Ns;        // o mod Ns
       const int g = fdiv(q, mR);       // o / (Ns R)
       const int r = q - g * R;         // (o / Ns) mod R
       const int j = g * Ns + k;
-      const int step = stride * (k + r * Ns);
+      const int step = K.stride * (k + r * Ns);
       double2 acc = cmk(0.0, 0.0);
       int t = 0;
       for (int m = 0; m < R; ++m) {
@@ -223,62 +235,61 @@ __device__ __forceinline__ void stage_generic(In in, Out out, int n, int R, int 
   if (to_lds) __syncthreads();
 }
 
-template <int MAXE, bool INV, class In, class Out>
-__device__ __forceinline__ void run_stage(int R, In in, Out out, int n, int Ns, const double2* __restrict__ tw,
+template <int MAXE, bool INV, bool FIRST, class In, class Out>
+__device__ __forceinline__ void run_stage(int R, In in, Out out, int n, StageK K, const double2* __restrict__ tw,
                                           int tid, int T) {
   switch (R) {
-    case 2: stage_r<2, (MAXE + 1) / 2, INV>(in, out, n, Ns, tw, tid, T); break;
-    case 3: stage_r<3, (MAXE + 2) / 3, INV>(in, out, n, Ns, tw, tid, T); break;
-    case 4: stage_r<4, (MAXE + 3) / 4, INV>(in, out, n, Ns, tw, tid, T); break;
-    case 5: stage_r<5, (MAXE + 4) / 5, INV>(in, out, n, Ns, tw, tid, T); break;
-    case 8: stage_r<8, (MAXE + 7) / 8, INV>(in, out, n, Ns, tw, tid, T); break;
-    default: stage_generic<MAXE, INV>(in, out, n, R, Ns, tw, tid, T); break;
+    case 2: stage_r<2, (MAXE + 1) / 2, INV, FIRST>(in, out, n, K, tw, tid, T); break;
+    case 3: stage_r<3, (MAXE + 2) / 3, INV, FIRST>(in, out, n, K, tw, tid, T); break;
+    case 4: stage_r<4, (MAXE + 3) / 4, INV, FIRST>(in, out, n, K, tw, tid, T); break;
+    case 5: stage_r<5, (MAXE + 4) / 5, INV, FIRST>(in, out, n, K, tw, tid, T); break;
+    case 8: stage_r<8, (MAXE + 7) / 8, INV, FIRST>(in, out, n, K, tw, tid, T); break;
+    default: stage_generic<MAXE, INV, FIRST>(in, out, n, R, K, tw, tid, T); break;
   }
 }
 
 // Full transform of the n values in(0..n-1) into out(0..n-1), with the plan
 // at Pg (device memory, read through the constant address space: the plan is
 // uniform); INV selects e^{+i}.  The first stage gathers straight from `in`
-// and the last scatters straight to `out`, so a transform whose source and
-// sink are global memory makes n-point LDS round trips only between stages;
-// buf (n double2) holds the intermediate stages.  A thread handles at most
-// MAXE values per stage: the launch uses T >= ceil(n/MAXE).  On return every
-// LDS read of the transform has completed in every thread only if `out` is
-// the LDS buffer (then the workgroup has synchronised); a caller that reuses
-// buf for its next transform with a global source needs no barrier: the
-// first stage synchronises before its scatter.
+// and the last scatters straight to `out`; buf (n double2 of LDS) holds the
+// stages between.  A thread handles at most MAXE values per stage: the
+// launch uses T >= ceil(n/MAXE).  A caller may start its next transform on
+// the same buf right away: every stage that writes buf synchronises the
+// workgroup before its scatter.
 template <int MAXE, bool INV, class In, class Out>
 __device__ __forceinline__ void lds_fft(Lds buf, const FftPlan* Pg, const double2* __restrict__ tw, int tid0, int T0,
                                         In in, Out out) {
-  constexpr bool in_lds = same_t<In, LdsIn>::value, out_lds = same_t<Out, LdsOut>::value;
-  typedef const __attribute__((address_space(4))) int32_t cint32;
-  const cint32* pc = (const cint32*)Pg;
-  const int n = pc[0];
-  const int nstage = pc[1];
+  typedef const __attribute__((address_space(4))) FftPlan cplan;
+  const cplan& P = *(const cplan*)Pg;
+  const int n = P.n;
+  const int nstage = P.nstage;
   if (nstage == 0) {  // n == 1
     if (tid0 == 0) out(0, in(0));
-    if (out_lds) __syncthreads();
     return;
   }
   const LdsIn li{buf};
   const LdsOut lo{buf};
-  int Ns = 1;
-  for (int s = 0; s < nstage; ++s) {
-    const int R = pc[2 + s];
-    const bool first = s == 0, last = s == nstage - 1;
-    // Opaque per-stage copies of tid and T: without them the compiler hoists
-    // every stage variant's index arithmetic out of the loop and keeps it
-    // live across all of them (doubling the VGPR count).
+  // Each stage takes opaque copies of tid and T: without them the compiler
+  // hoists every stage variant's index arithmetic out of the stage loop and
+  // keeps it live across all of them (doubling the VGPR count).
+  const StageK K0{1, P.stride[0], 0u};
+  {
     int tid = tid0, T = T0;
     asm volatile("" : "+v"(tid), "+s"(T));
-    if (!in_lds && first) {
-      if (!out_lds && last) run_stage<MAXE, INV>(R, in, out, n, Ns, tw, tid, T);
-      else run_stage<MAXE, INV>(R, in, lo, n, Ns, tw, tid, T);
-    } else if (!out_lds && last) {
-      run_stage<MAXE, INV>(R, li, out, n, Ns, tw, tid, T);
-    } else {
-      run_stage<MAXE, INV>(R, li, lo, n, Ns, tw, tid, T);
+    if (nstage == 1) {
+      run_stage<MAXE, INV, true>(P.radix[0], in, out, n, K0, tw, tid, T);
+      return;
     }
+    run_stage<MAXE, INV, true>(P.radix[0], in, lo, n, K0, tw, tid, T);
+  }
+  int Ns = P.radix[0];
+  for (int s = 1; s < nstage; ++s) {
+    const int R = P.radix[s];
+    const StageK K{Ns, P.stride[s], P.magic[s]};
+    int tid = tid0, T = T0;
+    asm volatile("" : "+v"(tid), "+s"(T));
+    if (s == nstage - 1) run_stage<MAXE, INV, false>(R, li, out, n, K, tw, tid, T);
+    else run_stage<MAXE, INV, false>(R, li, lo, n, K, tw, tid, T);
     Ns *= R;
   }
 }

@@ -24,16 +24,22 @@
 #define BLDFM_HD inline
 #endif
 
-#define BLDFM_MAX_RADIX_STAGES 24
+#define BLDFM_MAX_RADIX_STAGES 16
 #define BLDFM_LEVEL_STRIDE 16  // doubles per vertical step in the level table
 #define BLDFM_FFT_MAXE 10      // values per thread per FFT stage, 1024-thread kernels (n <= 10240)
 #define BLDFM_FFT_SMALL_N 2048 // longest transform of the 256-thread kernels (8 values per thread)
 
 // Mixed-radix Stockham plan for one transform length.
+// Stage s has radix radix[s] and current sub-transform length
+// Ns = radix[0]*...*radix[s-1]; stride[s] = n/(Ns*radix[s]) is its twiddle
+// index step and magic[s] = floor(2^32/Ns) + 1 divides by Ns with one
+// multiply-high (j/Ns = (j*magic) >> 32 for j < 2^16, Ns >= 2; 0 at s = 0).
 struct FftPlan {
   int32_t n;
   int32_t nstage;
   int32_t radix[BLDFM_MAX_RADIX_STAGES];
+  int32_t stride[BLDFM_MAX_RADIX_STAGES];
+  uint32_t magic[BLDFM_MAX_RADIX_STAGES];
   int64_t tw_off;  // offset (in double2) of the n twiddles e^{-2 pi i t/n}
 };
 

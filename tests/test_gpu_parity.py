@@ -197,7 +197,12 @@ def test_psi_phi_and_wind(gpu):
         assert abs(u - ou) <= 1e-15 * spd and abs(v - ov) <= 1e-15 * spd
 
 
-@pytest.mark.parametrize("shape", [(8, 6), (384, 960), (3, 2, 544, 192), (17, 11)])
+@pytest.mark.parametrize(
+    "shape",
+    # one-stage plans (5, 3, 17, 11, 7), a length-1 axis (no stage), radix 8/4/2/3/5 and a
+    # generic prime stage, and the 1024-thread kernels (10240 = 8*8*8*4*5)
+    [(8, 6), (5, 3), (7, 1), (384, 960), (3, 2, 544, 192), (17, 11), (2, 10240)],
+)
 @pytest.mark.parametrize("norm", ["backward", "forward", "ortho"])
 def test_fft2(gpu, shape, norm):
     from bldfm_amd import fft_manager
