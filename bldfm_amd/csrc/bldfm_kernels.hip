@@ -152,8 +152,11 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
     double2 sp1[MAXS], sq1[MAXS], sp2[MAXS], sq2[MAXS];
 #pragma unroll
     for (int s = 0; s < MAXS; ++s) sp1[s] = sq1[s] = sp2[s] = sq2[s] = cmk(0.0, 0.0);
-    // the next row is fetched while the current step computes
+    // the next row is fetched while the current step computes; unrolled by
+    // two so the prefetched row and the (p, q) state alternate between two
+    // register sets instead of being copied every step
     Row nxt = load_row(ctab);
+#pragma unroll 2
     for (int i = 0; i < nz - 1; ++i) {
       if (!RECOMP) {
         const int sl = slot[i];
