@@ -114,13 +114,19 @@ int max_lds_bytes(int device) {
   return v;
 }
 
-int check_fft_len(int n, int device, const char* what) {
-  const int64_t need = (int64_t)n * 16;
+// Longest transform of one workgroup: buffer and twiddle tables in LDS.
+int max_fft_len(int device) {
   const int lds = max_lds_bytes(device);
-  if (need > lds || fft_threads_for(n) > 1024)
+  int n = std::min(lds / 16, 1024 * kMaxE);
+  while (n > 1 && fft_lds_bytes(n) > lds) --n;
+  return n;
+}
+
+int check_fft_len(int n, int device, const char* what) {
+  if (n > max_fft_len(device))
     return fail(BLDFM_ELIMIT, std::string(what) + " = " + std::to_string(n) +
                                   " exceeds the single-workgroup FFT limit (" +
-                                  std::to_string(std::min(lds / 16, 1024 * kMaxE)) + " points)");
+                                  std::to_string(max_fft_len(device)) + " points)");
   return 0;
 }
 
@@ -496,7 +502,7 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
     for (int e = 0; e < 2; ++e) V.variant_used[k][e] = variant_used[k][e];
   V.fft_small = max_n <= BLDFM_FFT_SMALL_N ? 1 : 0;
   V.fft_threads = fft_threads_for(max_n);
-  V.fft_lds_bytes = max_n * (int)sizeof(double2);
+  V.fft_lds_bytes = fft_lds_bytes(max_n);  // non-decreasing in n
   V.tot_tab = tiles[0][ninst];
   V.tot_fwd_rows = tiles[1][ninst];
   V.tot_fwd_cols = tiles[2][ninst];
@@ -658,9 +664,9 @@ int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* 
   if (!rc) rc = cached_plan(device, n1, &F1, &tw1);
   if (rc) return rc;
   const int nmax = std::max(n0, n1);
-  set_lds_attr(nmax * 16);
+  set_lds_attr(fft_lds_bytes(nmax));
   rc = bldfm_launch_fft2(F0, n0, tw0, F1, n1, tw1, batch, (const double2*)in, (double2*)workspace,
-                         (double2*)out, sign, scale, fft_threads_for(nmax), nmax * 16, kGridCap, stream);
+                         (double2*)out, sign, scale, fft_threads_for(nmax), fft_lds_bytes(nmax), kGridCap, stream);
   if (rc) return fail(BLDFM_EHIP, std::string("fft2 launch failed: ") +
                                       hipGetErrorString((hipError_t)(-(rc + 1000))));
   return 0;

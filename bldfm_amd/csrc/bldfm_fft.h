@@ -108,18 +108,22 @@ __device__ __forceinline__ void butterfly<8>(double2* x, bool inv) {
 __device__ __forceinline__ uint64_t div_magic(int d) { return (1ull << 32) / (uint64_t)d + 1ull; }
 __device__ __forceinline__ int fdiv(int j, uint64_t m) { return (int)(((uint64_t)(uint32_t)j * m) >> 32); }
 
-// Element i of an LDS transform buffer lives at i ^ ((i >> 3) & 15) when the
-// length is a multiple of 16 (a bijection within every aligned block of 16),
-// else at i.  The first Stockham stages write with element strides R*Ns
-// (8, 16, 64, ...): unswizzled, 8 or 16 lanes of one ds_write_b128 would hit
-// the same banks; swizzled, every aligned group of 8 and of 16 lanes hits
-// distinct banks in every stage of the factorisations used here.
+// Element i of an LDS transform buffer lives at i ^ ((i >> 3) & 7) when the
+// length is a multiple of 8 (a bijection within every aligned block of 8),
+// else at i.  ds_read/write_b128 serve 8 lanes (128 bytes, all 32 banks) per
+// cycle; the first Stockham stages write with element strides R*Ns (8, 64,
+// ...) which, unswizzled, put all 8 lanes of a group on the same 4 banks.
+// Swizzled, every aligned group of 8 lanes hits distinct banks in every stage
+// of the factorisations used here.  The map only moves bits 3-5 into bits
+// 0-2, so phys(i + 64 m) = phys(i) + 64 m: an access pattern with a stride
+// that is a multiple of 64 needs one swizzle per thread, not per element.
 struct Lds {
   double2* p;
   int swz;
-  __device__ __forceinline__ double2& operator[](int i) const { return p[i ^ ((i >> 3) & swz)]; }
+  __device__ __forceinline__ int phys(int i) const { return i ^ ((i >> 3) & swz); }
+  __device__ __forceinline__ double2& operator[](int i) const { return p[phys(i)]; }
 };
-__device__ __forceinline__ Lds lds_view(double2* p, int n) { return Lds{p, (n & 15) ? 0 : 15}; }
+__device__ __forceinline__ Lds lds_view(double2* p, int n) { return Lds{p, (n & 7) ? 0 : 7}; }
 
 // Source and sink of the intermediate stages: the LDS buffer.
 struct LdsIn {
@@ -130,6 +134,28 @@ struct LdsOut {
   Lds b;
   __device__ __forceinline__ void operator()(int i, double2 v) const { b[i] = v; }
 };
+
+// Twiddles W_n^t from two short tables in LDS: with t = hi*L + lo (L = 2^lb,
+// L*L >= n), W^t = W^{hi*L} * W^{lo}.  Both factors are correctly rounded
+// table entries, so W^t is within a few ulp; the tables (L + ceil(n/L)
+// entries, 88 for n = 1536) sit behind the transform buffer and replace a
+// dependent global load per stage by two LDS reads (tw_split_log2 and
+// tw_table_len live with the plan types).
+struct TwLds {
+  const double2* lo;  // W^lo, lo < L
+  const double2* hi;  // W^{hi*L}, hi < ceil(n/L)
+  int lb;
+  __device__ __forceinline__ double2 operator()(int t) const {
+    return cmul(hi[t >> lb], lo[t & ((1 << lb) - 1)]);
+  }
+};
+// Fill the tables at dst from the n-entry global table; the caller
+// synchronises the workgroup before the first lookup.
+__device__ __forceinline__ TwLds tw_tables(double2* dst, const double2* __restrict__ tw, int n, int tid, int T) {
+  const int lb = tw_split_log2(n), L = 1 << lb, nhi = (n + L - 1) >> lb;
+  for (int i = tid; i < L + nhi; i += T) dst[i] = i < L ? tw[i < n ? i : 0] : tw[(i - L) << lb];
+  return TwLds{dst, dst + L, lb};
+}
 
 template <class A, class B> struct same_t { static constexpr bool value = false; };
 template <class A> struct same_t<A, A> { static constexpr bool value = true; };
@@ -147,7 +173,7 @@ struct StageK {
 // and after it (the next stage reads it).  FIRST: Ns = 1 (no twiddles,
 // k = 0, q = j).
 template <int R, int MAXB, bool INV, bool FIRST, class In, class Out>
-__device__ __forceinline__ void stage_r(In in, Out out, int n, StageK K, const double2* __restrict__ tw,
+__device__ __forceinline__ void stage_r(In in, Out out, int n, StageK K, TwLds tw,
                                         int tid, int T) {
   constexpr bool to_lds = same_t<Out, LdsOut>::value;
   const int nb = n / R;
@@ -156,14 +182,25 @@ __device__ __forceinline__ void stage_r(In in, Out out, int n, StageK K, const d
   for (int b = 0; b < MAXB; ++b) {
     const int j = tid + b * T;
     if (j < nb) {
+      if constexpr (same_t<In, LdsIn>::value) {
+        if ((nb & 63) == 0) {  // phys(j + r*nb) = phys(j) + r*nb
+          const int pj = in.b.phys(j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[b][r] = in(j + r * nb);
+          for (int r = 0; r < R; ++r) x[b][r] = in.b.p[pj + r * nb];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) x[b][r] = in(j + r * nb);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[b][r] = in(j + r * nb);
+      }
       if (!FIRST) {
         const int k = j - (int)__umulhi((unsigned)j, K.magic) * K.Ns;
         if (k != 0) {
           // W^{r k stride} for r = 1..R-1 from one table entry by repeated
           // multiplication (error grows like r*eps, not amplified downstream)
-          double2 w1 = tw[k * K.stride];
+          double2 w1 = tw(k * K.stride);
           if (INV) w1.y = -w1.y;
           double2 w = w1;
 #pragma unroll
@@ -188,8 +225,24 @@ __device__ __forceinline__ void stage_r(In in, Out out, int n, StageK K, const d
         const int q = (int)__umulhi((unsigned)j, K.magic);
         base = q * K.Ns * (R - 1) + j;  // q*Ns*R + k with k = j - q*Ns
       }
+      if constexpr (same_t<Out, LdsOut>::value) {
+        const Lds& L = out.b;
+        if (FIRST && R == 8) {  // element 8j + r: bits 3-5 are j & 7
+          const int jm = j & L.swz;
 #pragma unroll
-      for (int r = 0; r < R; ++r) out(base + r * (FIRST ? 1 : K.Ns), x[b][r]);
+          for (int r = 0; r < R; ++r) L.p[base + (r ^ jm)] = x[b][r];
+        } else if (!FIRST && (K.Ns & 63) == 0) {
+          const int pb = L.phys(base);
+#pragma unroll
+          for (int r = 0; r < R; ++r) L.p[pb + r * K.Ns] = x[b][r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) L[base + r * (FIRST ? 1 : K.Ns)] = x[b][r];
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) out(base + r * (FIRST ? 1 : K.Ns), x[b][r]);
+      }
     }
   }
   if (to_lds) __syncthreads();
@@ -198,7 +251,7 @@ __device__ __forceinline__ void stage_r(In in, Out out, int n, StageK K, const d
 // Any radix: one output per (thread, slot), out[o] = sum_m x_m W_n^{m*stride*(k + r*Ns)}.
 template <int MAXE, bool INV, bool FIRST, class In, class Out>
 __device__ __forceinline__ void stage_generic(In in, Out out, int n, int R, StageK K,
-                                              const double2* __restrict__ tw, int tid, int T) {
+                                              TwLds tw, int tid, int T) {
   constexpr bool to_lds = same_t<Out, LdsOut>::value;
   const int nb = n / R;
   const int Ns = FIRST ? 1 : K.Ns;
@@ -217,7 +270,7 @@ __device__ __forceinline__ void stage_generic(In in, Out out, int n, int R, Stag
       double2 acc = cmk(0.0, 0.0);
       int t = 0;
       for (int m = 0; m < R; ++m) {
-        double2 w = tw[t];
+        double2 w = tw(t);
         if (INV) w.y = -w.y;
         acc = cadd(acc, cmul(in(j + m * nb), w));
         t += step;
@@ -236,7 +289,7 @@ __device__ __forceinline__ void stage_generic(In in, Out out, int n, int R, Stag
 }
 
 template <int MAXE, bool INV, bool FIRST, class In, class Out>
-__device__ __forceinline__ void run_stage(int R, In in, Out out, int n, StageK K, const double2* __restrict__ tw,
+__device__ __forceinline__ void run_stage(int R, In in, Out out, int n, StageK K, TwLds tw,
                                           int tid, int T) {
   switch (R) {
     case 2: stage_r<2, (MAXE + 1) / 2, INV, FIRST>(in, out, n, K, tw, tid, T); break;
@@ -250,15 +303,18 @@ __device__ __forceinline__ void run_stage(int R, In in, Out out, int n, StageK K
 
 // Full transform of the n values in(0..n-1) into out(0..n-1), with the plan
 // at Pg (device memory, read through the constant address space: the plan is
-// uniform); INV selects e^{+i}.  The first stage gathers straight from `in`
-// and the last scatters straight to `out`; buf (n double2 of LDS) holds the
-// stages between.  A thread handles at most MAXE values per stage: the
-// launch uses T >= ceil(n/MAXE).  A caller may start its next transform on
-// the same buf right away: every stage that writes buf synchronises the
-// workgroup before its scatter.
+// uniform) and its n-entry twiddle table tw (global); INV selects e^{+i}.
+// The first stage gathers straight from `in` and the last scatters straight
+// to `out`; buf holds the stages between (n double2) followed by the two
+// twiddle tables (tw_table_len(n) double2).  A thread handles at most MAXE
+// values per stage: the launch uses T >= ceil(n/MAXE).  `reuse`: this
+// workgroup transformed on buf before (its last stage may still be reading
+// the tables, so the workgroup synchronises before refilling them); the
+// transform buffer itself needs no such care, since every stage that writes
+// buf synchronises the workgroup before its scatter.
 template <int MAXE, bool INV, class In, class Out>
-__device__ __forceinline__ void lds_fft(Lds buf, const FftPlan* Pg, const double2* __restrict__ tw, int tid0, int T0,
-                                        In in, Out out) {
+__device__ __forceinline__ void lds_fft(Lds buf, const FftPlan* Pg, const double2* __restrict__ twg, int tid0,
+                                        int T0, In in, Out out, bool reuse) {
   typedef const __attribute__((address_space(4))) FftPlan cplan;
   const cplan& P = *(const cplan*)Pg;
   const int n = P.n;
@@ -266,6 +322,12 @@ __device__ __forceinline__ void lds_fft(Lds buf, const FftPlan* Pg, const double
   if (nstage == 0) {  // n == 1
     if (tid0 == 0) out(0, in(0));
     return;
+  }
+  if (reuse) __syncthreads();
+  const TwLds tw = tw_tables(buf.p + n, twg, n, tid0, T0);
+  {  // the radix-2/3/4/5/8 first stage reads no twiddle before its barrier
+    const int R0 = P.radix[0];
+    if (R0 != 2 && R0 != 3 && R0 != 4 && R0 != 5 && R0 != 8) __syncthreads();
   }
   const LdsIn li{buf};
   const LdsOut lo{buf};

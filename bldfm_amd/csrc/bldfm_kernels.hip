@@ -504,12 +504,13 @@ struct SplitOut {
   }
 };
 
+// One transform of a tile loop; `reuse`: not the workgroup's first tile.
 template <int MAXE, class In, class Out>
 __device__ __forceinline__ void fft_dir(bool inv, const cplan_t& F, const double2* __restrict__ tw, int tid, int T,
-                                        In in, Out out) {
+                                        In in, Out out, bool reuse) {
   const bldfm_fft::Lds buf = bldfm_fft::lds_view(smem, F.n);
-  if (inv) bldfm_fft::lds_fft<MAXE, true>(buf, (const FftPlan*)&F, tw + F.tw_off, tid, T, in, out);
-  else bldfm_fft::lds_fft<MAXE, false>(buf, (const FftPlan*)&F, tw + F.tw_off, tid, T, in, out);
+  if (inv) bldfm_fft::lds_fft<MAXE, true>(buf, (const FftPlan*)&F, tw + F.tw_off, tid, T, in, out, reuse);
+  else bldfm_fft::lds_fft<MAXE, false>(buf, (const FftPlan*)&F, tw + F.tw_off, tid, T, in, out, reuse);
 }
 
 // Forward, rows: one surface-flux row y placed at [px, px+nx) of nxe points,
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
     const PadIn<double> in{B.src + D.off_src + (int64_t)y * D.nx, 1, D.px, D.nx};
     const BandOut out{B.ws + D.off_aw + (int64_t)y * D.nlx, 1.0 / (double)F.n, Band(D.nlx, F.n)};
-    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out);
+    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out, b != blockIdx.x);
   }
 }
 
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
     const PadIn<double2> in{B.ws + D.off_aw + jx, D.nlx, D.py, D.ny};
     const BandOut out{B.ws + D.off_q0 + (int64_t)jx * D.nly, 1.0 / (double)F.n, Band(D.nly, F.n)};
-    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out);
+    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out, b != blockIdx.x);
   }
 }
 
@@ -561,7 +562,7 @@ __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
     const BandIn in{B.ws + D.off_z + local * D.nyb, 1, Band(D.nyb, F.n)};
     const CropOut out{B.ws + D.off_v + local * D.ny, D.py, D.ny};
-    fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in, out);
+    fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in, out, b != blockIdx.x);
   }
 }
 
@@ -583,10 +584,10 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
     const double cm = (B.mode && B.mode[tr.inst] == 0) ? 0.0 : 1.0;
     if (D.out_f32)
       fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in,
-                    SplitOut<float>{(float*)B.conc + o, (float*)B.flx + o, cm, D.px, D.nx});
+                    SplitOut<float>{(float*)B.conc + o, (float*)B.flx + o, cm, D.px, D.nx}, b != blockIdx.x);
     else
       fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in,
-                    SplitOut<double>{(double*)B.conc + o, (double*)B.flx + o, cm, D.px, D.nx});
+                    SplitOut<double>{(double*)B.conc + o, (double*)B.flx + o, cm, D.px, D.nx}, b != blockIdx.x);
   }
 }
 
@@ -708,10 +709,10 @@ struct StridedOut {
 
 template <int MAXE>
 __device__ __forceinline__ void fft_plain(bool inv, const FftPlan* F, int n, const double2* __restrict__ tw, int tid,
-                                          int T, StridedIn in, StridedOut out) {
+                                          int T, StridedIn in, StridedOut out, bool reuse) {
   const bldfm_fft::Lds buf = bldfm_fft::lds_view(smem, n);
-  if (inv) bldfm_fft::lds_fft<MAXE, true>(buf, F, tw, tid, T, in, out);
-  else bldfm_fft::lds_fft<MAXE, false>(buf, F, tw, tid, T, in, out);
+  if (inv) bldfm_fft::lds_fft<MAXE, true>(buf, F, tw, tid, T, in, out, reuse);
+  else bldfm_fft::lds_fft<MAXE, false>(buf, F, tw, tid, T, in, out, reuse);
 }
 
 template <int MAXE, int MAXT>
@@ -720,7 +721,8 @@ __global__ __launch_bounds__(MAXT) void k_fft2_rows(const FftPlan* F, const doub
                                                   int32_t n, int64_t nrows, bool inv) {
   const int T = blockDim.x, tid = threadIdx.x;
   for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x)
-    fft_plain<MAXE>(inv, F, n, tw, tid, T, StridedIn{in + r * n, 1}, StridedOut{mid + r * n, 1, 1.0});
+    fft_plain<MAXE>(inv, F, n, tw, tid, T, StridedIn{in + r * n, 1}, StridedOut{mid + r * n, 1, 1.0},
+                    r != blockIdx.x);
 }
 
 template <int MAXE, int MAXT>
@@ -731,7 +733,8 @@ __global__ __launch_bounds__(MAXT) void k_fft2_cols(const FftPlan* F, const doub
   for (int64_t c = blockIdx.x; c < ncols; c += gridDim.x) {
     const int64_t b = c / n1, col = c % n1;
     const int64_t o = b * (int64_t)n0 * n1 + col;
-    fft_plain<MAXE>(inv, F, n0, tw, tid, T, StridedIn{mid + o, n1}, StridedOut{out + o, n1, scale});
+    fft_plain<MAXE>(inv, F, n0, tw, tid, T, StridedIn{mid + o, n1}, StridedOut{out + o, n1, scale},
+                    c != blockIdx.x);
   }
 }
 

@@ -26,7 +26,7 @@
 
 #define BLDFM_MAX_RADIX_STAGES 16
 #define BLDFM_LEVEL_STRIDE 16  // doubles per vertical step in the level table
-#define BLDFM_FFT_MAXE 10      // values per thread per FFT stage, 1024-thread kernels (n <= 10240)
+#define BLDFM_FFT_MAXE 10      // values per thread per FFT stage, 1024-thread kernels (n <= 10033: 160 KB of LDS)
 #define BLDFM_FFT_SMALL_N 2048 // longest transform of the 256-thread kernels (8 values per thread)
 
 // Mixed-radix Stockham plan for one transform length.
@@ -42,6 +42,20 @@ struct FftPlan {
   uint32_t magic[BLDFM_MAX_RADIX_STAGES];
   int64_t tw_off;  // offset (in double2) of the n twiddles e^{-2 pi i t/n}
 };
+
+// Twiddle tables of an LDS transform of length n: W^lo (lo < L = 2^lb,
+// L*L >= n) and W^{hi*L} (hi < ceil(n/L)), behind the n-point buffer.
+BLDFM_HD int tw_split_log2(int n) {
+  int lb = 0;
+  while ((1 << (2 * lb)) < n) ++lb;
+  return lb;
+}
+BLDFM_HD int tw_table_len(int n) {
+  const int lb = tw_split_log2(n);
+  return (1 << lb) + ((n + (1 << lb) - 1) >> lb);
+}
+// Dynamic LDS of one transform: buffer and twiddle tables.
+BLDFM_HD int fft_lds_bytes(int n) { return (n + tw_table_len(n)) * 16; }
 
 // One tile of a kernel's work: the instance and the tile's index inside it.
 struct TileRec {

@@ -200,8 +200,8 @@ def test_psi_phi_and_wind(gpu):
 @pytest.mark.parametrize(
     "shape",
     # one-stage plans (5, 3, 17, 11, 7), a length-1 axis (no stage), radix 8/4/2/3/5 and a
-    # generic prime stage, and the 1024-thread kernels (10240 = 8*8*8*4*5)
-    [(8, 6), (5, 3), (7, 1), (384, 960), (3, 2, 544, 192), (17, 11), (2, 10240)],
+    # generic prime stage, and the 1024-thread kernels near the LDS limit (10000 = 8*2*5^4)
+    [(8, 6), (5, 3), (7, 1), (384, 960), (3, 2, 544, 192), (17, 11), (2, 10000)],
 )
 @pytest.mark.parametrize("norm", ["backward", "forward", "ortho"])
 def test_fft2(gpu, shape, norm):
@@ -213,6 +213,15 @@ def test_fft2(gpu, shape, norm):
         got = ours(x, norm=norm)
         exp = ref(x, norm=norm)
         assert rel_l2(got.real, exp.real) <= 1e-13 and rel_l2(got.imag, exp.imag) <= 1e-13
+
+
+def test_fft2_length_limit(gpu):
+    """A transform that cannot hold its buffer and twiddle tables in one
+    workgroup's LDS is refused with a clear error, not run."""
+    from bldfm_amd import _lib, fft_manager
+
+    with pytest.raises((ValueError, _lib.BLDFMError), match="single-workgroup FFT limit"):
+        fft_manager.fft2(np.ones((2, 10240), dtype=complex))
 
 
 def test_device_math_rounds_like_host(gpu):
