@@ -17,6 +17,7 @@ a bounded sample of the same workload on one host core.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                     [--surrogate CHECKPOINT] [--tol 1e-3] [--no-surrogate]
+    python bench.py --copy-calibration   # 1 GiB device copy, the PMC byte reference
 
 The surrogate leg serves the same batch with the trained network
 (``bldfm_amd/data``), re-solving 1 % of every footprint's modes in FP64 as
@@ -192,6 +193,22 @@ def cpu_baseline(specs, gpu_batch, nsample):
     return nsample / t, worst
 
 
+def copy_calibration():
+    """A plain device copy of 1 GiB (three times): profiled with rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE, it gives the factors that turn those
+    counters into HBM bytes."""
+    import torch
+
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,7 +220,12 @@ def main():
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "bldfm_amd", "data", "surrogate_most_n26.pt"))
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(batches shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
