@@ -16,7 +16,7 @@ events on the launch stream, and a CPU baseline: the numpy oracle timed on
 a bounded sample of the same workload on one host core.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-                    [--surrogate CHECKPOINT] [--tol 1e-3] [--no-surrogate]
+                    [--surrogate CHECKPOINT] [--tol 1e-3] [--no-surrogate] [--no-tune]
     python bench.py --copy-calibration   # 1 GiB device copy, the PMC byte reference
 
 The surrogate leg serves the same batch with the trained network
@@ -106,6 +106,7 @@ def timed(batch_obj, steps, warmup):
 
 MFMA_FLOP_PER_POINT = 2 * (96 * 256 + 4 * 256 * 256)  # bf16 MFMA work of one network evaluation
 PEAK_BF16 = 2.5e15  # MI355X dense bf16 MFMA
+PEAK_HBM = 8.0e12  # MI355X HBM3E bytes/s
 
 
 def surrogate_leg(specs, fb, args):
@@ -120,7 +121,7 @@ def surrogate_leg(specs, fb, args):
     if not os.path.exists(args.surrogate):
         return {"error": f"no trained network at {args.surrogate}"}
     net = sur.SurrogateNet.load(args.surrogate)
-    fbs = FootprintBatch(specs, precision="double", fast=False, surrogate=net, tol=args.tol)
+    fbs = FootprintBatch(specs, precision="double", fast=False, surrogate=net, tol=args.tol, tuning=fb.tuning)
     elapsed, phases, n = timed(fbs, args.steps, args.warmup)
     fps = len(specs) * args.steps / elapsed
     err, mode = fbs.surrogate_stats()
@@ -193,6 +194,29 @@ def cpu_baseline(specs, gpu_batch, nsample):
     return nsample / t, worst
 
 
+def fft_roofline(fb, phases, n):
+    """HBM roofline of the two inverse passes: algorithmic bytes (band in,
+    cropped columns out; cropped rows in, conc and flx out) over their
+    HIP-event time."""
+    def band(nl, ne):
+        return nl + 1 if (nl % 2 == 0 and nl < ne) else nl
+
+    by = bx = 0
+    for inst in fb.instances:
+        nxe, nye = inst.nx + 2 * inst.px, inst.ny + 2 * inst.py
+        nxb, nyb = band(inst.nlx, nxe), band(inst.nly, nye)
+        ob = 4 if inst.out_f32 else 8
+        by += inst.nslot * nxb * (nyb + inst.ny) * 16
+        bx += inst.nslot * inst.ny * (nxb * 16 + inst.nx * 2 * ob)
+    ty = phases["inverse_y"] / max(n, 1) * 1e-3
+    tx = phases["inverse_x"] / max(n, 1) * 1e-3
+    out = {"bound": "hbm", "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
+    for name, b, t in (("k_inv_cols", by, ty), ("k_inv_rows", bx, tx)):
+        out[name] = {"bytes": b, "ms": round(t * 1e3, 4), "achieved": round(b / t / 1e9, 1),
+                     "frac": round(b / t / PEAK_HBM, 4)}
+    return out
+
+
 def copy_calibration():
     """A plain device copy of 1 GiB (three times): profiled with rocprofv3
     --pmc FETCH_SIZE / WRITE_SIZE, it gives the factors that turn those
@@ -220,6 +244,7 @@ def main():
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "bldfm_amd", "data", "surrogate_most_n26.pt"))
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuned one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -238,7 +263,11 @@ def main():
     torch.cuda.set_device(0)
     specs = workload(args.batch)
     print(f"[bench] batch={args.batch} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-    fb = FootprintBatch(specs, precision="double", fast=False)
+    # launch shape chosen by the autotuner (timed before, outside the measured steps)
+    fb = FootprintBatch(specs, precision="double", fast=False, tuning=None if args.no_tune else "auto")
+    if fb.tune_report is not None:
+        print(f"[bench] autotuned launch shape {fb.tuning}: "
+              + ", ".join(f"{k} {v:.3f} ms" for k, v in fb.tune_report["ms"].items()), file=sys.stderr, flush=True)
     elapsed, phases, n = timed(fb, args.steps, args.warmup)
     fps = args.batch * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
@@ -250,7 +279,7 @@ def main():
 
     extra = {}
     if not args.no_fast:
-        fbf = FootprintBatch(specs, precision="double", fast=True)
+        fbf = FootprintBatch(specs, precision="double", fast=True, tuning=fb.tuning)
         el_f, ph_f, n_f = timed(fbf, args.steps, args.warmup)
         extra["fast_fma_value"] = round(args.batch * args.steps / el_f, 2)
         extra["fast_fma_shoot_ms"] = round(ph_f["shoot"] / max(n_f, 1), 4)
@@ -306,6 +335,10 @@ def main():
             "issue_ceiling_frac_without_fma": 0.5,
         },
         "phase_ms_per_step": {k: round(v / max(n, 1), 4) for k, v in phases.items()},
+        "fft_roofline": fft_roofline(fb, phases, n),
+        "tuning": {"choice": fb.tuning,
+                   "candidates_ms": ({k: round(v, 4) for k, v in fb.tune_report["ms"].items()}
+                                     if fb.tune_report is not None else None)},
         "cpu_baseline": {
             "value": round(cpu_fps, 4) if cpu_fps is not None else None,
             "unit": "footprints/s",

@@ -108,6 +108,8 @@ SIGNATURES = {
     "bldfm_profile_query": (ctypes.c_int, [ctypes.POINTER(Met), ctypes.POINTER(_I32), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "bldfm_profiles": (ctypes.c_int, [ctypes.POINTER(Met), _P, _P]),
     "bldfm_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Instance), _I32, ctypes.POINTER(_I32), ctypes.POINTER(_P)]),
+    "bldfm_plan_create_tuned": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Instance), _I32, ctypes.POINTER(_I32),
+                                               ctypes.c_void_p, ctypes.POINTER(_P)]),
     "bldfm_plan_destroy": (ctypes.c_int, [_P]),
     "bldfm_plan_sizes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "bldfm_plan_offsets": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
@@ -213,17 +215,39 @@ def make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height,
     return m
 
 
-class Plan:
-    """Owns a ``bldfm_plan``; sizes and per-instance offsets are cached."""
+class Tuning(ctypes.Structure):
+    """``bldfm_tuning``: launch shape of a plan (see bldfm_amd.autotune)."""
 
-    def __init__(self, instances, levels_i32, dev=None):
+    _fields_ = [("shoot_block", _I32), ("xpass_layout", _I32), ("reserved", _I32 * 6)]
+
+    def as_dict(self):
+        return {"shoot_block": self.shoot_block or 256, "xpass_layout": self.xpass_layout}
+
+
+def make_tuning(shoot_block=256, xpass_layout=0):
+    t = Tuning()
+    t.shoot_block = int(shoot_block)
+    t.xpass_layout = int(xpass_layout)
+    return t
+
+
+class Plan:
+    """Owns a ``bldfm_plan``; sizes and per-instance offsets are cached.
+    ``tuning`` (a dict with shoot_block / xpass_layout, or a Tuning) sets
+    the launch shape; None keeps the defaults."""
+
+    def __init__(self, instances, levels_i32, dev=None, tuning=None):
         lib = load()
         self.device = dev if dev is not None else device()
         n = len(instances)
         arr = (Instance * n)(*instances)
         lv = (ctypes.c_int32 * len(levels_i32))(*[int(v) for v in levels_i32])
         handle = ctypes.c_void_p()
-        check(lib.bldfm_plan_create(self.device.index, arr, n, lv, ctypes.byref(handle)), "plan_create")
+        if isinstance(tuning, dict):
+            tuning = make_tuning(**tuning)
+        self.tuning = tuning.as_dict() if tuning is not None else make_tuning().as_dict()
+        tp = ctypes.byref(tuning) if tuning is not None else None
+        check(lib.bldfm_plan_create_tuned(self.device.index, arr, n, lv, tp, ctypes.byref(handle)), "plan_create")
         self.handle = handle
         ws = ctypes.c_size_t()
         prof, src, out = _I64(), _I64(), _I64()

@@ -113,6 +113,10 @@ def _instance(spec: FootprintSpec, precision: str, fast: bool, surrogate: bool =
 class FootprintBatch:
     """A batch of footprint solves sharing one plan and one set of buffers.
 
+    ``tuning``: launch shape of the plan -- None for the defaults, a dict of
+    ``bldfm_tuning`` fields, or "auto" to time the candidates on the device
+    first (``bldfm_amd.autotune``; the choice is cached per batch shape).
+
     With ``surrogate`` (a trained ``bldfm_amd.surrogate.SurrogateNet``), each
     execution serves the instances inside the network's training hull from
     the network, re-solves one mode in 100 of each in FP64, and solves an
@@ -120,7 +124,7 @@ class FootprintBatch:
     ``tol``.  Served instances produce the flux footprint (conc is 0)."""
 
     def __init__(self, specs: Sequence[FootprintSpec], precision: str = "double", fast: bool = False,
-                 surrogate=None, tol: float = 1e-3):
+                 surrogate=None, tol: float = 1e-3, tuning=None):
         if len(specs) == 0:
             raise ValueError("empty batch")
         self.specs = list(specs)
@@ -128,7 +132,17 @@ class FootprintBatch:
         built = [_instance(s, precision, fast, surrogate is not None) for s in self.specs]
         self.instances = [b[0] for b in built]
         ints = np.concatenate([b[1] for b in built]).astype(np.int32)
-        self.plan = _lib.Plan(self.instances, ints, self.device)
+        self.levels_i32 = ints
+        self.tune_report = None
+        if isinstance(tuning, str):
+            if tuning != "auto":
+                raise ValueError("tuning must be None, 'auto' or a dict of bldfm_tuning fields")
+            from . import autotune
+
+            self.tune_report = autotune.tune(self.instances, ints, self.device)
+            tuning = {k: self.tune_report[k] for k in ("shoot_block", "xpass_layout")}
+        self.plan = _lib.Plan(self.instances, ints, self.device, tuning=tuning)
+        self.tuning = self.plan.tuning
         dev = self.device
         self.prof = torch.empty(max(self.plan.prof_doubles, 1), dtype=torch.float64, device=dev)
         self.conc = torch.empty(max(self.plan.out_elems, 1), dtype=torch.float64, device=dev)

@@ -269,8 +269,18 @@ int bldfm_profiles(const bldfm_met* met, double* prof6_dev, hipStream_t stream) 
 
 int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, const int32_t* levels,
                       bldfm_plan** out) {
+  return bldfm_plan_create_tuned(device, inst, ninst, levels, nullptr, out);
+}
+
+int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t ninst, const int32_t* levels,
+                            const bldfm_tuning* tuning, bldfm_plan** out) {
   if (!inst || ninst <= 0 || !levels || !out) return fail(BLDFM_EINVAL, "bad plan arguments");
   *out = nullptr;
+  const int shoot_block = tuning && tuning->shoot_block ? tuning->shoot_block : 256;
+  const int xlayout = tuning ? tuning->xpass_layout : 0;
+  if (shoot_block != 64 && shoot_block != 128 && shoot_block != 256)
+    return fail(BLDFM_EINVAL, "shoot_block must be 64, 128 or 256");
+  if (xlayout < 0 || xlayout > 2) return fail(BLDFM_EINVAL, "xpass_layout must be 0, 1 or 2");
   DeviceGuard guard(device);
   if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
 
@@ -288,7 +298,8 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
     plan_n.push_back(n);
     return (int)plan_n.size() - 1;
   };
-  constexpr int kKinds = 9;  // tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows, sur, spot
+  // tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows, sur, spot, transpose
+  constexpr int kKinds = 10;
   std::vector<int64_t> tiles[kKinds];
   for (auto& v : tiles) v.assign(ninst + 1, 0);
   std::vector<SurMet> surmet(ninst);
@@ -399,6 +410,7 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
     D.off_z = off_ws; off_ws += (int64_t)D.nslot * D.nxb * D.nyb;
     D.off_nyq = off_ws; off_ws += (int64_t)D.nslot * (D.nlx + D.nly) * 2;
     D.off_v = off_ws; off_ws += (int64_t)D.nslot * D.nxb * D.ny;
+    if (xlayout == 2) { D.off_vt = off_ws; off_ws += (int64_t)D.nslot * D.nxb * D.ny; }
     D.off_out = off_out; off_out += (int64_t)D.nslot * D.ny * D.nx;
 
     // tiles: tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows
@@ -408,12 +420,13 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
         (D.nz + 255) / 256,
         D.footprint ? 0 : D.ny,
         D.footprint ? 0 : D.nlx,
-        (D.lanes + 255) / 256,
+        (D.lanes + shoot_block - 1) / shoot_block,
         fixc > 0 ? (int64_t)D.nslot * ((fixc + 255) / 256) : 0,
         (int64_t)D.nslot * D.nxb,
         (int64_t)D.nslot * D.ny,
         want_sur ? (D.lanes + SUR_PTS - 1) / SUR_PTS : 0,
         want_sur ? (nspot + 255) / 256 : 0,
+        xlayout == 2 ? (int64_t)D.nslot * ((D.nxb + 31) / 32) * ((D.ny + 31) / 32) : 0,
     };
     for (int k = 0; k < kKinds; ++k) tiles[k][i + 1] = tiles[k][i] + nt[k];
   }
@@ -489,6 +502,10 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst, con
   V.t_inv_rows = (const TileRec*)(base + o_tiles[6]);
   V.t_sur = (const TileRec*)(base + o_tiles[7]);
   V.t_spot = (const TileRec*)(base + o_tiles[8]);
+  V.t_tr = (const TileRec*)(base + o_tiles[9]);
+  V.tot_tr = tiles[9][ninst];
+  V.shoot_block = shoot_block;
+  V.xlayout = xlayout;
   V.sur_met = (const SurMet*)(base + o_surmet);
   V.sur_ok = nullptr;
   V.sur_net = nullptr;

@@ -28,6 +28,11 @@ struct PlanDev {
   int32_t fft_small;       // all transforms <= BLDFM_FFT_SMALL_N: 256-thread kernels
   int32_t fft_threads;     // threads per FFT workgroup
   int32_t fft_lds_bytes;   // dynamic LDS per FFT workgroup
+  int32_t shoot_block;     // modes per workgroup of the vertical solve (64, 128, 256)
+  int32_t xlayout;         // x-pass input: 0 strided gather, 1 y-pass writes transposed,
+                           // 2 tiled transpose kernel between the passes
+  const TileRec* t_tr;     // 32x32 tiles of the transpose (xlayout 2)
+  int64_t tot_tr;
   int64_t tot_tab, tot_fwd_rows, tot_fwd_cols, tot_shoot, tot_nyq, tot_inv_cols, tot_inv_rows;
   // learned surrogate of footprint instances (unused while sur_net is null)
   const TileRec* t_sur;    // SUR_PTS-lane tiles of surrogate-flagged instances

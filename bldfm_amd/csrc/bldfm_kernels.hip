@@ -18,7 +18,6 @@
 #include "bldfm_fft.h"
 #include "bldfm_march.h"
 
-#define SHOOT_THREADS 256
 #define FFT_SMALL_E 8     // values per thread per stage, 256-thread FFT kernels (n <= 2048)
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -128,15 +127,15 @@ __device__ void origin_numeric(const cdesc_t& D, const double* __restrict__ tab,
 // Numerical branch.  MAXS register snapshots (RECOMP == false), or a second
 // march from the bottom after the shooting coefficient is known
 // (RECOMP == true) for many output levels.  Both give the same values.
-template <int MAXS, bool RECOMP, bool EXACT>
-__global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, int variant) {
+template <int MAXS, bool RECOMP, bool EXACT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shoot(PlanDev P, ExecBufs B, int variant) {
   for (int64_t t = blockIdx.x; t < P.tot_shoot; t += gridDim.x) {
     const TileRec tr = tile_rec(P.t_shoot, t);
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
     if (D.variant != variant || D.exact != (int)EXACT) continue;
     if (B.mode && B.mode[tr.inst] == 0) continue;  // accepted surrogate instance
     Lane L;
-    const int lane = tr.local * SHOOT_THREADS + (int)threadIdx.x;
+    const int lane = tr.local * BLOCK + (int)threadIdx.x;
     if (!lane_setup(D, lane, L)) continue;
     const double* __restrict__ tab = B.tab + D.off_tab;
     ctab_t* __restrict__ ctab = (ctab_t*)tab;
@@ -207,7 +206,8 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot(PlanDev P, ExecBufs B, 
 
 // Analytic branch for constant profiles (src/bldfm/solver.py:193-202):
 // q^ = q^0 e^{-lambda h}, p^ = (q^ / Kz) / lambda, p^00 = p000 - (q^00 / Kz) h.
-__global__ __launch_bounds__(SHOOT_THREADS) void k_shoot_analytic(PlanDev P, ExecBufs B, int variant) {
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shoot_analytic(PlanDev P, ExecBufs B, int variant) {
 #pragma clang fp contract(off)
   for (int64_t t = blockIdx.x; t < P.tot_shoot; t += gridDim.x) {
     const TileRec tr = tile_rec(P.t_shoot, t);
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(SHOOT_THREADS) void k_shoot_analytic(PlanDev P, Exe
     if (D.variant != variant) continue;
     if (B.mode && B.mode[tr.inst] == 0) continue;
     Lane L;
-    const int lane = tr.local * SHOOT_THREADS + (int)threadIdx.x;
+    const int lane = tr.local * BLOCK + (int)threadIdx.x;
     if (!lane_setup(D, lane, L)) continue;
     const int nz = D.nz;
     const double* tt = B.tab + D.off_tab + (int64_t)(nz - 1) * BLDFM_LEVEL_STRIDE;
@@ -478,13 +478,14 @@ struct PadIn {
   }
 };
 
-// Outputs [off, off + cnt) to a contiguous array.
+// Outputs [off, off + cnt) to an array with element stride `stride`.
 struct CropOut {
   double2* __restrict__ dst;
+  int64_t stride;
   int off, cnt;
   __device__ __forceinline__ void operator()(int o, double2 v) const {
     const unsigned t = (unsigned)(o - off);
-    if (t < (unsigned)cnt) dst[t] = v;
+    if (t < (unsigned)cnt) dst[(int64_t)t * stride] = v;
   }
 };
 
@@ -561,7 +562,10 @@ __global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
     const int64_t local = tr.local;  // s*nxb + cx
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
     const BandIn in{B.ws + D.off_z + local * D.nyb, 1, Band(D.nyb, F.n)};
-    const CropOut out{B.ws + D.off_v + local * D.ny, D.py, D.ny};
+    // layout 1: column cx of slot s lands at v[s][y][cx]
+    const int64_t s = local / D.nxb, cx = local % D.nxb;
+    const CropOut out = P.xlayout == 1 ? CropOut{B.ws + D.off_v + s * D.ny * D.nxb + cx, D.nxb, D.py, D.ny}
+                                       : CropOut{B.ws + D.off_v + local * D.ny, 1, D.py, D.ny};
     fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in, out, b != blockIdx.x);
   }
 }
@@ -578,7 +582,10 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
     const int64_t local = tr.local;  // s*ny + y
     const int s = (int)(local / D.ny), y = (int)(local % D.ny);
     const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
-    const BandIn in{B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, Band(D.nxb, F.n)};
+    const BandIn in = P.xlayout == 0
+                          ? BandIn{B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, Band(D.nxb, F.n)}
+                          : BandIn{B.ws + (P.xlayout == 1 ? D.off_v : D.off_vt) + ((int64_t)s * D.ny + y) * D.nxb,
+                                   1, Band(D.nxb, F.n)};
     const int64_t o = D.off_out + local * D.nx;
     // instances served by the surrogate carry no concentration spectrum
     const double cm = (B.mode && B.mode[tr.inst] == 0) ? 0.0 : 1.0;
@@ -588,6 +595,35 @@ __global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
     else
       fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in,
                     SplitOut<double>{(double*)B.conc + o, (double*)B.flx + o, cm, D.px, D.nx}, b != blockIdx.x);
+  }
+}
+
+// x-pass layout 2: v[s][cx][y] -> vt[s][y][cx] in 32 x 32 tiles through LDS.
+// Reads run along y and writes along cx, both coalesced; the padding column
+// makes the transposed LDS reads (stride 33 x 16 bytes) conflict free.
+__global__ __launch_bounds__(256) void k_transpose_v(PlanDev P, ExecBufs B) {
+  __shared__ double2 tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int64_t b = blockIdx.x; b < P.tot_tr; b += gridDim.x) {
+    const TileRec tr = tile_rec(P.t_tr, b);
+    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
+    const int tcx = (D.nxb + 31) >> 5, tcy = (D.ny + 31) >> 5;
+    const int s = tr.local / (tcx * tcy), r = tr.local % (tcx * tcy);
+    const int cx0 = (r / tcy) * 32, y0 = (r % tcy) * 32;
+    const double2* src = B.ws + D.off_v + (int64_t)s * D.nxb * D.ny;
+    double2* dst = B.ws + D.off_vt + (int64_t)s * D.ny * D.nxb;
+    if (b != blockIdx.x) __syncthreads();  // the previous tile's reads of `tile` are done
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cx = cx0 + ty + 8 * k, y = y0 + tx;
+      if (cx < D.nxb && y < D.ny) tile[ty + 8 * k][tx] = src[(int64_t)cx * D.ny + y];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int y = y0 + ty + 8 * k, cx = cx0 + tx;
+      if (cx < D.nxb && y < D.ny) dst[(int64_t)y * D.nxb + cx] = tile[tx][ty + 8 * k];
+    }
   }
 }
 
@@ -643,10 +679,12 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
   }
   MARK(3)
   const unsigned gs = grid_for(P.tot_shoot, grid_cap);
-#define SHOOT(V, MAXS, RECOMP, EX)                                                    \
-  if (P.variant_used[V][EX]) {                                                        \
-    k_shoot<MAXS, RECOMP, EX><<<gs, SHOOT_THREADS, 0, stream>>>(P, B, V);             \
-    LAUNCH_CHECK();                                                                   \
+#define SHOOT(V, MAXS, RECOMP, EX)                                                           \
+  if (P.variant_used[V][EX]) {                                                               \
+    if (P.shoot_block == 64) k_shoot<MAXS, RECOMP, EX, 64><<<gs, 64, 0, stream>>>(P, B, V);     \
+    else if (P.shoot_block == 128) k_shoot<MAXS, RECOMP, EX, 128><<<gs, 128, 0, stream>>>(P, B, V); \
+    else k_shoot<MAXS, RECOMP, EX, 256><<<gs, 256, 0, stream>>>(P, B, V);                     \
+    LAUNCH_CHECK();                                                                          \
   }
   SHOOT(0, 1, false, true)
   SHOOT(1, 4, false, true)
@@ -656,7 +694,9 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
   SHOOT(2, 1, true, false)
 #undef SHOOT
   if (P.variant_used[3][0] || P.variant_used[3][1]) {
-    k_shoot_analytic<<<gs, SHOOT_THREADS, 0, stream>>>(P, B, 3);
+    if (P.shoot_block == 64) k_shoot_analytic<64><<<gs, 64, 0, stream>>>(P, B, 3);
+    else if (P.shoot_block == 128) k_shoot_analytic<128><<<gs, 128, 0, stream>>>(P, B, 3);
+    else k_shoot_analytic<256><<<gs, 256, 0, stream>>>(P, B, 3);
     LAUNCH_CHECK();
   }
   MARK(4)
@@ -667,6 +707,10 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
   MARK(5)
   FFT_LAUNCH(k_inv_cols, P.tot_inv_cols)
   MARK(6)
+  if (P.xlayout == 2 && P.tot_tr > 0) {
+    k_transpose_v<<<grid_for(P.tot_tr, grid_cap), 256, 0, stream>>>(P, B);
+    LAUNCH_CHECK();
+  }
   FFT_LAUNCH(k_inv_rows, P.tot_inv_rows)
   MARK(7)
 #undef FFT_LAUNCH

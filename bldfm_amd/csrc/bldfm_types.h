@@ -105,7 +105,9 @@ struct InstDesc {
   int64_t off_q0;       // workspace double2: q^0 band, x-major (nlx*nly)
   int64_t off_z;        // workspace double2: packed Z, (nslot, nxb, nyb)
   int64_t off_nyq;      // workspace double2: Nyquist modes, (nslot, nlx+nly, 2)
-  int64_t off_v;        // workspace double2: after y-pass, (nslot, nxb, ny)
+  int64_t off_v;        // workspace double2: after y-pass, (nslot, nxb, ny), or (nslot, ny, nxb)
+                        // when the plan's x-pass layout is 1
+  int64_t off_vt;       // workspace double2: (nslot, ny, nxb) transpose of v (x-pass layout 2)
   int64_t off_out;      // output element offset: (nslot, ny, nx)
 };
 

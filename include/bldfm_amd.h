@@ -137,6 +137,27 @@ int bldfm_wind_fields(const double* speed, const double* dir, int64_t n, double*
 int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst,
                       const int32_t* levels, bldfm_plan** plan);
 
+/*
+ * Launch shape of a plan.  No counterpart in the reference, whose numba and
+ * FFTW back ends choose their own; bldfm_amd.autotune picks it by timing the
+ * candidates on the device (bldfm_plan_create uses the defaults).
+ *   shoot_block   modes per workgroup of the vertical solve: 64, 128 or 256
+ *                 (0 selects 256)
+ *   xpass_layout  how the x-pass reads the y-pass output: 0 gathers each row
+ *                 at stride ny; 1 the y-pass stores its columns transposed;
+ *                 2 a coalesced LDS-tiled transpose runs between the passes
+ *                 (one more y-pass-sized workspace block)
+ * Every choice gives the same results (bit for bit).
+ */
+typedef struct {
+  int32_t shoot_block;
+  int32_t xpass_layout;
+  int32_t reserved[6];
+} bldfm_tuning;
+
+int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t ninst,
+                            const int32_t* levels, const bldfm_tuning* tuning, bldfm_plan** plan);
+
 int bldfm_plan_destroy(bldfm_plan* plan);
 
 /* Buffer sizes of a plan: workspace bytes, profile doubles (6*nz summed),

@@ -118,3 +118,21 @@ def test_cli_dry_run(tmp_path, caplog):
         assert main(["run", str(p), "--dry-run"]) == 0
     text = caplog.text
     assert "Dry run complete" in text and "Tower 'T'" in text and "Timesteps: 1" in text
+
+
+def test_autotune_shape_key():
+    """The autotuner's cache key ignores instance order and tracks geometry."""
+    import torch
+
+    from bldfm_amd import _lib, autotune
+
+    def inst(nx, nz, footprint=1):
+        i = _lib.Instance()
+        i.nx, i.ny, i.nlx, i.nly, i.nz, i.nslot, i.footprint = nx, nx, nx, nx, nz, 1, footprint
+        return i
+
+    cpu = torch.device("cpu")
+    a = [inst(64, 20), inst(128, 30)]
+    assert autotune.shape_key(a, cpu) == autotune.shape_key(list(reversed(a)), cpu)
+    assert autotune.shape_key(a, cpu) != autotune.shape_key([inst(64, 20), inst(128, 31)], cpu)
+    assert autotune.shape_key(a, cpu) != autotune.shape_key([inst(64, 20), inst(128, 30, 0)], cpu)
