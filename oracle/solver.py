@@ -130,12 +130,17 @@ def steady_state_transport_solver(
     *,
     _mutation=None,
     _return_spectra=False,
+    _alpha_ulps=0,
 ):
     """CPU restatement of src/bldfm/solver.py:16-304.
 
     Returns ``((X, Y, Z), conc, flx)`` squeezed, like the reference.  With
     ``_return_spectra`` the truncated (nlvls, nly, nlx) spectra p^, q^ after
-    the shift are returned as well.
+    the shift are returned as well.  ``_alpha_ulps`` (tests only) scales the
+    shooting coefficient by 1 + k*2^-52 before the combination: the change of
+    the output measures how strongly a level depends on the last bit of
+    alpha (near the top of a tall 3-D field the growing and decaying
+    solutions cancel, and that dependence exceeds any fixed tolerance).
     """
     assert _mutation in VALID_MUTATIONS
     q0 = np.asarray(srf_flx)
@@ -211,6 +216,8 @@ def steady_state_transport_solver(
             alpha = -(q2 + kzt * eig * p2) / (q1 + kzt * eig * p1)
         else:
             alpha = -(q2 - kzt * eig * p2) / (q1 - kzt * eig * p1)
+        if _alpha_ulps:
+            alpha = alpha * (1.0 + _alpha_ulps * 2.0**-52)
         tp[:, msk] = alpha * sp1 + sp2
         tq[:, msk] = alpha * sq1 + sq2
         # degenerate mode (0,0): trapezoid in z, flux conserved (solver.py:239-251)

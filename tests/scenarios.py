@@ -106,6 +106,17 @@ SCENARIOS = {
         dict(nxy=(512, 512), domain=(2000.0, 2000.0), levels=26, modes=(512, 512), meas_pt=(1000.0, 1000.0), footprint=True, precision="double"),
         ("zeros",),
     ),
+    # C2 3-D dispersion: 16 seeded Gaussians (sigma U(20,150) m, amplitude U(0.5,2)), every level
+    "c2_unstable_dispersion": (
+        dict(n=26, meas_height=15.0, wind=met_wind(c2_speed(-20.0), 270.0), ustar=0.3, mol=-20.0, domain_height=100.0),
+        dict(nxy=(512, 512), domain=(2000.0, 2000.0), levels=list(range(65)), modes=(512, 512), precision="double"),
+        ("gaussians", 20261208, 16, 20.0, 150.0, 0.5, 2.0),
+    ),
+    "c2_stable_dispersion": (
+        dict(n=26, meas_height=15.0, wind=met_wind(c2_speed(100.0), 270.0), ustar=0.3, mol=100.0, domain_height=100.0),
+        dict(nxy=(512, 512), domain=(2000.0, 2000.0), levels=list(range(65)), modes=(512, 512), precision="double"),
+        ("gaussians", 20261208, 16, 20.0, 150.0, 0.5, 2.0),
+    ),
 }
 
 

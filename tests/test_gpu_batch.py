@@ -17,17 +17,17 @@ from oracle import solver as osolver
 pytestmark = pytest.mark.gpu
 
 
-def c3_like_specs(count, seed=20261205):
+def c3_like_specs(count, seed=20261205, grids=(32, 64, 128), ns=(8, 16, 32)):
     """Seeded instances drawn like the batched-solver workload (grid from a
-    set, n from a set, u*, 1/L, wind speed, direction, zm uniform), at
-    sizes the oracle finishes in seconds."""
+    set, n from a set, u*, 1/L, wind speed, direction, zm uniform); the
+    default sets are sizes the oracle finishes in seconds."""
     from bldfm_amd.batch import FootprintSpec
 
     rng = np.random.default_rng(seed)
     specs = []
     while len(specs) < count:
-        g = int(rng.choice([32, 64, 128]))
-        n = int(rng.choice([8, 16, 32]))
+        g = int(rng.choice(grids))
+        n = int(rng.choice(ns))
         ustar = rng.uniform(0.1, 1.0)
         inv_l = rng.uniform(-0.1, 0.05)
         mol = 1.0 / inv_l if inv_l != 0 else 1e9
@@ -106,3 +106,38 @@ def test_batch_level_beyond_grid_raises():
                          meas_height=10.0, wind=(0.0, -5.0), ustar=0.4, levels=[400])
     with pytest.raises(IndexError):
         FootprintBatch([spec])
+
+
+def test_c3_full_size_graph_batch_equals_single_solves():
+    """The batched-solver workload at full size: 256 footprints drawn from
+    grids {128^2, 256^2, 512^2} and n {16, 32, 64} with seeded meteorology,
+    one plan, captured in a HIP graph and replayed; every instance equals its
+    own single-instance solve bit for bit, and the two smallest agree with
+    the oracle to 1e-10."""
+    import torch
+
+    from bldfm_amd.batch import FootprintBatch
+
+    specs = c3_like_specs(256, seed=20261209, grids=(128, 256, 512), ns=(16, 32, 64))
+    print(f"[c3 full] grids {sorted(set(s.nx for s in specs))}, n {sorted(set(s.n for s in specs))}", flush=True)
+    fb = FootprintBatch(specs, precision="double")
+    fb.capture()
+    fb.replay()
+    fb.replay()
+    torch.cuda.synchronize()
+    for i, spec in enumerate(specs):
+        one = FootprintBatch([spec], precision="double")
+        one.run()
+        torch.cuda.synchronize()
+        for a, b in zip(fb.output(i), one.output(0)):
+            assert torch.equal(a, b), f"instance {i}"
+        del one
+    small = [i for i, s in enumerate(specs) if s.nx == 128][:2]
+    for i in small:
+        s = specs[i]
+        z, profs = fb.profiles(i)  # device profiles (pinned against the oracle's in the test above)
+        _, oc, of = osolver.steady_state_transport_solver(
+            np.zeros((s.ny, s.nx)), z, profs, s.domain, s.n, modes=s.modes, meas_pt=s.meas_pt, footprint=True,
+            precision="double")
+        gc, gf = (t.cpu().numpy() for t in fb.output(i))
+        assert rel_l2(gf.squeeze(), of) <= 1e-10 and rel_l2(gc.squeeze(), oc) <= 1e-10
