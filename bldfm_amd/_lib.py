@@ -108,6 +108,7 @@ SIGNATURES = {
     "bldfm_profile_query": (ctypes.c_int, [ctypes.POINTER(Met), ctypes.POINTER(_I32), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "bldfm_profiles": (ctypes.c_int, [ctypes.POINTER(Met), _P, _P]),
     "bldfm_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Instance), _I32, ctypes.POINTER(_I32), ctypes.POINTER(_P)]),
+    "bldfm_plan_update_met": (ctypes.c_int, [_P, ctypes.POINTER(Met), _I32, ctypes.c_void_p]),
     "bldfm_plan_create_tuned": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Instance), _I32, ctypes.POINTER(_I32),
                                                ctypes.c_void_p, ctypes.POINTER(_P)]),
     "bldfm_plan_destroy": (ctypes.c_int, [_P]),
@@ -281,6 +282,13 @@ class Plan:
         n = _I32()
         check(load().bldfm_plan_timing_end(self.handle, ms, ctypes.byref(n)), "timing_end")
         return dict(zip(self.PHASES, list(ms))), n.value
+
+    def update_met(self, mets):
+        """Replace the meteorology of the plan's instances (a list of Met,
+        one per instance; each keeps its number of vertical points)."""
+        n = len(mets)
+        arr = (Met * n)(*mets)
+        check(load().bldfm_plan_update_met(self.handle, arr, n, stream_handle(self.device)), "plan_update_met")
 
     def set_surrogate(self, net, hull, tol):
         """Serve surrogate-flagged instances with the network buffer ``net``

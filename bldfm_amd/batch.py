@@ -153,6 +153,33 @@ class FootprintBatch:
         if surrogate is not None:
             self.plan.set_surrogate(surrogate.net, surrogate.hull.c_struct(), tol)
 
+    _GEOMETRY = ("nx", "ny", "domain", "modes", "meas_pt", "halo", "levels")
+
+    def update(self, specs: Sequence[FootprintSpec]):
+        """New meteorology (wind, u*, z0, L, measurement height, ...) for the
+        batch's instances, same geometry (grid, domain, modes, tower
+        position, halo, requested levels) and the same number of vertical
+        points each: the plan, its buffers and a captured graph are reused.
+        Raises ValueError when a spec changes the geometry or the vertical
+        grid size."""
+        specs = list(specs)
+        if len(specs) != len(self.specs):
+            raise ValueError("update needs one spec per instance of the batch")
+        for i, (new, old) in enumerate(zip(specs, self.specs)):
+            for f in self._GEOMETRY:
+                if getattr(new, f) != getattr(old, f):
+                    raise ValueError(f"instance {i}: update may not change {f}")
+            if old.levels is None and new.n != old.n:
+                raise ValueError(f"instance {i}: update may not change n (the output level)")
+            if new.closure in ("CONSTANT", "MOST", "MOSTM") and new.ustar is not None and new.z0 is not None:
+                raise ValueError("Either z0 or ustar must be provided.")
+        mets = [_lib.make_met(s.n, s.meas_height, s.wind, s.ustar, s.z0, s.mol, s.prsc, s.closure, s.domain_height,
+                              s.stretch, s.tke) for s in specs]
+        self.plan.update_met(mets)
+        for inst, m in zip(self.instances, mets):
+            inst.met = m
+        self.specs = specs
+
     def surrogate_stats(self):
         """After a run: per instance the spot-check relative L2 error (NaN
         when outside the hull) and the path taken (0 surrogate, 1 solver)."""

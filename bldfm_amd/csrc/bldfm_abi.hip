@@ -183,6 +183,7 @@ struct bldfm_plan {
   int sur_any = 0;
   int64_t off_gate = 0, off_mode = 0;
   void* sur_ok_dev = nullptr;
+  bldfm_sur_hull hull{};  // hull of the surrogate set last (for bldfm_plan_update_met)
 };
 
 // Network inputs of one met setup (layout of SurMet); n is the level index
@@ -850,6 +851,46 @@ int bldfm_plan_set_surrogate(bldfm_plan* P, const void* net, const bldfm_sur_hul
   P->dev.sur_ok = (const int32_t*)P->sur_ok_dev;
   P->dev.sur_net = (const uint8_t*)net;
   P->dev.sur_tol = tol;
+  P->hull = *hull;
+  return 0;
+}
+
+int bldfm_plan_update_met(bldfm_plan* P, const bldfm_met* mets, int32_t n, hipStream_t stream) {
+  if (!P || !mets || n != P->ninst) return fail(BLDFM_EINVAL, "update_met needs one met per plan instance");
+  DeviceGuard guard(P->device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  // validate everything before changing anything
+  std::vector<ProfSetup> setups = P->setups;
+  for (int i = 0; i < n; ++i) {
+    if (!P->inst[i].prof_from_met) continue;
+    ProfSetup s;
+    const int rc = met_setup(&mets[i], &s);
+    if (rc) return rc;
+    if (s.nlev != P->desc[i].nz)
+      return fail(BLDFM_EINVAL, "instance " + std::to_string(i) + ": the new meteorology changes the vertical grid (" +
+                                    std::to_string(P->desc[i].nz) + " -> " + std::to_string(s.nlev) +
+                                    " levels); build a new plan");
+    setups[i] = s;
+  }
+  std::vector<SurMet> surmet(n);
+  for (int i = 0; i < n; ++i) {
+    memset(&surmet[i], 0, sizeof(SurMet));
+    if (!P->inst[i].prof_from_met) continue;
+    P->inst[i].met = mets[i];
+    if (P->inst[i].surrogate && P->desc[i].footprint) surmet[i] = sur_met_of(setups[i], mets[i].n);
+  }
+  P->setups = setups;
+  // stream-ordered after earlier executions, which read these arrays
+  hipError_t e = hipMemcpyAsync((void*)P->dev.met, setups.data(), sizeof(ProfSetup) * n, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && P->sur_any)
+    e = hipMemcpyAsync((void*)P->dev.sur_met, surmet.data(), sizeof(SurMet) * n, hipMemcpyHostToDevice, stream);
+  std::vector<int32_t> ok(n, 0);
+  if (e == hipSuccess && P->dev.sur_net && P->sur_ok_dev) {
+    for (int i = 0; i < n; ++i) ok[i] = sur_in_hull(P, i, P->hull) ? 1 : 0;
+    e = hipMemcpyAsync(P->sur_ok_dev, ok.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);  // the host arrays go out of scope
+  if (e != hipSuccess) return hip_fail(e, "uploading new meteorology");
   return 0;
 }
 

@@ -160,6 +160,19 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
 
 int bldfm_plan_destroy(bldfm_plan* plan);
 
+/*
+ * New meteorology for the instances of an existing plan (no counterpart in
+ * the reference, which rebuilds everything per call): mets[i] replaces the
+ * met of instance i when it is a prof_from_met instance (ignored otherwise).
+ * Every instance must keep its number of vertical points, so the tile
+ * tables and buffer layout stay valid; otherwise BLDFM_EINVAL and the plan
+ * is unchanged.  The upload is ordered on `stream` after earlier executions
+ * and the call returns once it is done; graphs captured from the plan replay
+ * with the new values (serving loops reuse one plan and one graph).  The
+ * surrogate's hull flags are re-evaluated against the hull last set.
+ */
+int bldfm_plan_update_met(bldfm_plan* plan, const bldfm_met* mets, int32_t n, hipStream_t stream);
+
 /* Buffer sizes of a plan: workspace bytes, profile doubles (6*nz summed),
  * source doubles (ny*nx summed over dispersion instances), output elements
  * (nslot*ny*nx summed). */
