@@ -91,3 +91,39 @@ def test_invalid_shapes_are_refused(specs, bad):
 
     with pytest.raises((ValueError, _lib.BLDFMError)):
         FootprintBatch(specs[:2], tuning=bad)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_layouts_on_dispersion_solves(precision):
+    """The x-pass layouts (including the LDS-tiled transpose) and block sizes
+    on dispersion instances (e^{+i} transforms) with several output levels,
+    in FP64 and with float32 outputs: bit-identical to the default shape."""
+    import torch
+
+    from bldfm_amd import _lib, solver
+    from bldfm_amd.pbl_model import vertical_profiles
+    from bldfm_amd.utils import ideal_source
+
+    dev = _lib.device()
+    nx, ny, dom, levels = 96, 80, (480.0, 400.0), [2, 7, 11]
+    z, profs = vertical_profiles(12, 8.0, (3.0, -2.0), ustar=0.35, mol=-80.0)
+    q0 = ideal_source((nx, ny), dom, shape="diamond")
+    g = solver._geometry((ny, nx), np.asarray(z), dom, levels, (nx, ny), (0.0, 0.0), False, None, precision)
+    slot, idx = solver._level_maps(g["levels"], g["nz"])
+    ints = np.concatenate([slot, idx]).astype(np.int32)
+    inst = solver._instance(g, len(levels), False, False, precision, 0.0)
+    prof = torch.as_tensor(np.concatenate([np.asarray(z)] + [np.asarray(p) for p in profs]), device=dev)
+    src = torch.as_tensor(np.ascontiguousarray(q0, dtype=np.float64), device=dev).reshape(-1)
+    odt = torch.float32 if g["out_f32"] else torch.float64
+    outs = []
+    for tuning in [None] + CANDIDATES:
+        plan = _lib.Plan([inst], ints, dev, tuning=tuning)
+        conc = torch.empty(plan.out_elems, dtype=odt, device=dev)
+        flx = torch.empty_like(conc)
+        ws = torch.empty(max(plan.workspace_bytes, 1), dtype=torch.uint8, device=dev)
+        plan.execute(prof, src, conc, flx, ws)
+        torch.cuda.synchronize()
+        outs.append((conc, flx))
+    assert torch.isfinite(outs[0][0]).all() and float(outs[0][0].abs().max()) > 0
+    for (c, f), t in zip(outs[1:], CANDIDATES):
+        assert torch.equal(c, outs[0][0]) and torch.equal(f, outs[0][1]), t
