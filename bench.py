@@ -252,8 +252,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(batches shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
+              "(footprint batches would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     import torch
