@@ -369,10 +369,12 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     // levels: slot map then level index per slot
     D.off_slot = (int64_t)ints.size();
     int nfilled = 0, max_level = -1;
+    for (int k = 0; k < 4; ++k) D.snap_level[k] = -1;
     for (int z = 0; z < D.nz; ++z) {
       const int s = lv[z];
       if (s >= 0) {
         if (s != nfilled || s >= D.nslot) return bad(BLDFM_EINVAL, "level->slot map must number slots 0,1,.. in ascending level order");
+        if (s < 4) D.snap_level[s] = z;
         ++nfilled;
         max_level = z;
       }

@@ -153,29 +153,28 @@ __global__ __launch_bounds__(BLOCK) void k_shoot(PlanDev P, ExecBufs B, int vari
     for (int s = 0; s < MAXS; ++s) sp1[s] = sq1[s] = sp2[s] = sq2[s] = cmk(0.0, 0.0);
     // the next row is fetched while the current step computes; unrolled by
     // two so the prefetched row and the (p, q) state alternate between two
-    // register sets instead of being copied every step
+    // register sets instead of being copied every step.  The snapshot levels
+    // come from the descriptor (uniform compares), so no step waits on a
+    // load issued in the same step.
+    int snap[MAXS];
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s) snap[s] = D.snap_level[s];
     Row nxt = load_row(ctab);
 #pragma unroll 2
     for (int i = 0; i < nz - 1; ++i) {
       if (!RECOMP) {
-        const int sl = slot[i];
-        if (sl >= 0) {
 #pragma unroll
-          for (int s = 0; s < MAXS; ++s)
-            if (s == sl) { sp1[s] = p1; sq1[s] = q1; sp2[s] = p2; sq2[s] = q2; }
-        }
+        for (int s = 0; s < MAXS; ++s)
+          if (i == snap[s]) { sp1[s] = p1; sq1[s] = q1; sp2[s] = p2; sq2[s] = q2; }
       }
       const Row cur = nxt;
       if (i + 1 < nz - 1) nxt = load_row(ctab + (int64_t)(i + 1) * BLDFM_LEVEL_STRIDE);
       march_step<EXACT>(cur, lx, ly, lx2, ly2, p1, q1, p2, q2);
     }
     if (!RECOMP) {
-      const int sl = slot[nz - 1];
-      if (sl >= 0) {
 #pragma unroll
-        for (int s = 0; s < MAXS; ++s)
-          if (s == sl) { sp1[s] = p1; sq1[s] = q1; sp2[s] = p2; sq2[s] = q2; }
-      }
+      for (int s = 0; s < MAXS; ++s)
+        if (nz - 1 == snap[s]) { sp1[s] = p1; sq1[s] = q1; sp2[s] = p2; sq2[s] = q2; }
     }
     const double* tt = tab + (int64_t)(nz - 1) * BLDFM_LEVEL_STRIDE;
     const double2 alpha = shoot_alpha(tt, top_lambda(tt, lx, ly, lx2, ly2), p1, q1, p2, q2);

@@ -90,6 +90,7 @@ struct InstDesc {
   int32_t variant;         // shooting kernel: 0 one register snapshot, 1 up to four,
                            // 2 two-pass march, 3 analytic
   int32_t exact;           // 1: reference rounding (no fused multiply-add) in the march
+  int32_t snap_level[4];   // level of output slot s < 4 of the numerical branch, or -1
   double dx, dy;
   double sx, sy;        // wavenumber scales ((2 pi)/dx)/nxe, ((2 pi)/dy)/nye
   double vx, vy;        // fftfreq value factors 1/(nl*(1/nl))
