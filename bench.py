@@ -13,10 +13,12 @@ step evaluates profiles, marches every mode, shoots, and transforms back.
 Prints one JSON line (the driver's contract) with the roofline of the
 dominant kernel (the shooting kernel, FP64 VALU bound) measured with HIP
 events on the launch stream, and a CPU baseline: the numpy oracle timed on
-a bounded sample of the same workload on one host core.
+a bounded sample of the same workload (16 footprints) in 16 worker processes,
+one footprint per host core at a time, like the reference's process pool.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                     [--surrogate CHECKPOINT] [--tol 1e-3] [--no-surrogate] [--no-tune]
+                    [--cpu-sample 16] [--cpu-workers 16]
     python bench.py --copy-calibration   # 1 GiB device copy, the PMC byte reference
 
 The surrogate leg serves the same batch with the trained network
@@ -165,33 +167,51 @@ def surrogate_leg(specs, fb, args):
     }
 
 
-def cpu_baseline(specs, gpu_batch, nsample):
-    """Oracle (numpy restatement of the reference algorithm) on one core, on
-    the first ``nsample`` footprints of the workload; also returns the
-    largest rel-L2 between those GPU footprints and the oracle's."""
-    from oracle import pbl as opbl
+def _oracle_footprint(job):
+    """One footprint by the oracle (numpy restatement of the reference
+    algorithm): the CPU baseline's unit of work, run in a worker process."""
+    ny, nx, domain, n, modes, meas_pt, z, profs = job
     from oracle import solver as osolver
 
-    worst = 0.0
-    t = 0.0
+    t0 = time.perf_counter()
+    _, oc, of = osolver.steady_state_transport_solver(
+        np.zeros((ny, nx)), z, profs, domain, n, modes=modes, meas_pt=meas_pt, footprint=True, precision="double",
+    )
+    return oc, of, time.perf_counter() - t0
+
+
+def cpu_baseline(specs, gpu_batch, nsample, workers, pool_ctx=None):
+    """Oracle footprints on `workers` host cores (one footprint per worker
+    process at a time, like the reference's process-pool fan-out), on the
+    first ``nsample`` footprints of the workload: footprints per second of
+    wall time.  Also returns the largest rel-L2 between those GPU footprints
+    and the oracle's."""
+    from oracle import pbl as opbl
+
+    jobs = []
     for i in range(nsample):
         s = specs[i]
         z, profs = gpu_batch.profiles(i)  # same inputs for both solvers
-        t0 = time.perf_counter()
-        _, oc, of = osolver.steady_state_transport_solver(
-            np.zeros((s.ny, s.nx)), z, profs, s.domain, s.n, modes=s.modes, meas_pt=s.meas_pt,
-            footprint=True, precision="double",
-        )
-        t += time.perf_counter() - t0
-        gc, gf = gpu_batch.output(i)
-        gc, gf = gc[0].cpu().numpy(), gf[0].cpu().numpy()
-        for a, b in ((gc, oc), (gf, of)):
-            worst = max(worst, float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+        jobs.append((s.ny, s.nx, s.domain, s.n, s.modes, s.meas_pt, z, profs))
         # profiles: device evaluation vs the oracle's
         oz, _ = opbl.vertical_profiles(s.n, s.meas_height, s.wind, ustar=s.ustar, mol=s.mol,
                                        domain_height=s.domain_height)
         assert len(oz) == len(z)
-    return nsample / t, worst
+    t0 = time.perf_counter()
+    if workers > 1:
+        with pool_ctx.Pool(workers) as pool:
+            res = pool.map(_oracle_footprint, jobs, chunksize=1)
+    else:
+        res = [_oracle_footprint(j) for j in jobs]
+    wall = time.perf_counter() - t0
+    worst = 0.0
+    for i, (oc, of, _) in enumerate(res):
+        gc, gf = gpu_batch.output(i)
+        gc, gf = gc[0].cpu().numpy(), gf[0].cpu().numpy()
+        for a, b in ((gc, oc), (gf, of)):
+            worst = max(worst, float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+    cpu_s = sum(r[2] for r in res)
+    return nsample / wall, worst, cpu_s
 
 
 def fft_roofline(fb, phases, n):
@@ -239,7 +259,9 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=128)
-    ap.add_argument("--cpu-sample", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=16, help="footprints of the CPU baseline (0: skip it)")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="worker processes (host cores) of the CPU baseline; 1 runs it in this process")
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "bldfm_amd", "data", "surrogate_most_n26.pt"))
     ap.add_argument("--tol", type=float, default=1e-3)
@@ -255,6 +277,17 @@ def main():
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(footprint batches would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
+    workers = max(1, min(args.cpu_workers, args.cpu_sample)) if args.cpu_sample > 0 else 1
+    pool_ctx = None
+    if workers > 1:
+        # The baseline's worker processes are forked by a fork server started
+        # here, before this process touches the GPU: no process that has
+        # initialised the GPU forks or execs.
+        import multiprocessing as mp
+        import multiprocessing.forkserver as mpfs
+
+        pool_ctx = mp.get_context("forkserver")
+        mpfs.ensure_running()
 
     import torch
 
@@ -299,7 +332,11 @@ def main():
                   f"FP64 solver), {sl['accepted']} accepted / {sl['fallback']} fallback", file=sys.stderr, flush=True)
         extra["surrogate"] = sl
 
-    cpu_fps, worst = cpu_baseline(specs, fb, args.cpu_sample) if args.cpu_sample > 0 else (None, None)
+    cpu_fps, worst, cpu_s = (cpu_baseline(specs, fb, args.cpu_sample, workers, pool_ctx) if args.cpu_sample > 0
+                             else (None, None, None))
+    if cpu_fps is not None:
+        print(f"[bench] cpu baseline: {cpu_fps:.3f} footprints/s on {workers} cores "
+              f"({args.cpu_sample} footprints, {cpu_s:.1f} s of CPU time)", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -342,9 +379,11 @@ def main():
         "cpu_baseline": {
             "value": round(cpu_fps, 4) if cpu_fps is not None else None,
             "unit": "footprints/s",
-            "cores": 1,
+            "cores": workers,
             "kind": "port",
-            "sample": f"{args.cpu_sample} footprints of the same workload (numpy oracle, one core)",
+            "sample": (f"{args.cpu_sample} footprints of the same workload (numpy oracle, {workers} worker "
+                       f"process{'es' if workers > 1 else ''}, one footprint each at a time; "
+                       f"{cpu_s:.1f} s of CPU time)" if cpu_fps is not None else None),
         },
         "parity": {"max_rel_l2_vs_oracle": worst, "footprints_checked": args.cpu_sample},
     }
