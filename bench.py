@@ -277,6 +277,7 @@ def main():
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(footprint batches would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
+    args.cpu_sample = max(0, min(args.cpu_sample, args.batch))
     workers = max(1, min(args.cpu_workers, args.cpu_sample)) if args.cpu_sample > 0 else 1
     pool_ctx = None
     if workers > 1:
