@@ -219,16 +219,19 @@ def make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height,
 class Tuning(ctypes.Structure):
     """``bldfm_tuning``: launch shape of a plan (see bldfm_amd.autotune)."""
 
-    _fields_ = [("shoot_block", _I32), ("xpass_layout", _I32), ("reserved", _I32 * 6)]
+    _fields_ = [("shoot_block", _I32), ("xpass_layout", _I32), ("fft_threads", _I32), ("reserved", _I32 * 5)]
 
     def as_dict(self):
-        return {"shoot_block": self.shoot_block or 256, "xpass_layout": self.xpass_layout}
+        return {"shoot_block": self.shoot_block or 256, "xpass_layout": self.xpass_layout,
+                "fft_threads": self.fft_threads}
 
 
-def make_tuning(shoot_block=256, xpass_layout=0):
+def make_tuning(shoot_block=256, xpass_layout=0, fft_threads=0):
+    """fft_threads 0: the plan's default workgroup size for its FFT passes."""
     t = Tuning()
     t.shoot_block = int(shoot_block)
     t.xpass_layout = int(xpass_layout)
+    t.fft_threads = int(fft_threads)
     return t
 
 

@@ -9,9 +9,14 @@ the work is cut into workgroups, never a result:
 * ``xpass_layout``: how the x-pass reads the y-pass output -- 0 gathers each
   row at stride ny, 1 has the y-pass store its columns transposed, 2 runs a
   coalesced LDS-tiled transpose kernel between the passes.
+* ``fft_threads``: threads per workgroup of the FFT passes (0: one thread per
+  8 or 10 points of the longest transform, rounded up to a wave).  More
+  threads leave some idle in the radix-8 stages but split the short-radix
+  stages over more waves.
 
 ``tune`` times each candidate on the device with the batch's own instances
-(coordinate search: block size first, then layout, so 3 + 2 plans), keeps
+(coordinate search: block size, then layout, then FFT workgroup size, so
+3 + 2 + up to 2 plans), keeps
 the fastest (a candidate has to win by more than 1%, above run-to-run noise),
 and caches the choice per (device, batch shape) in memory and,
 when a path is given, in a JSON file, so a serving loop pays the search once
@@ -30,7 +35,7 @@ from . import _lib
 
 SHOOT_BLOCKS = (256, 128, 64)
 XPASS_LAYOUTS = (0, 1, 2)
-DEFAULT = {"shoot_block": 256, "xpass_layout": 0}
+DEFAULT = {"shoot_block": 256, "xpass_layout": 0, "fft_threads": 0}
 
 _FIELDS = ("nx", "ny", "nlx", "nly", "px", "py", "nz", "nslot", "footprint", "analytic", "single",
            "out_f32", "fast", "surrogate", "prof_from_met")
@@ -46,6 +51,23 @@ def shape_key(instances, dev=None):
     rows = sorted(tuple(int(getattr(inst, f)) for f in _FIELDS) for inst in instances)
     h = hashlib.sha256(json.dumps([name, rows]).encode()).hexdigest()[:24]
     return f"{name}:{len(rows)}:{h}"
+
+
+def fft_thread_candidates(instances):
+    """FFT workgroup sizes worth timing for these instances: the default (0)
+    and up to two larger multiples of 64 within the kernel family's bound
+    (256 threads for transforms up to 2048 points, else 1024)."""
+    n = max(max(int(i.nx) + 2 * int(i.px), int(i.ny) + 2 * int(i.py)) for i in instances)
+    small = n <= 2048
+    per = 8 if small else 10
+    base = max(64, -(-(-(-n // per)) // 64) * 64)  # ceil(n/per) rounded up to a multiple of 64
+    cap = 256 if small else 1024
+    out = [0]
+    for t in (base + 64, 2 * base):
+        t = -(-t // 64) * 64
+        if base < t <= cap and t not in out:
+            out.append(t)
+    return out
 
 
 def _time_plan(instances, levels, dev, tuning, reps, warmup, prof, src):
@@ -97,16 +119,17 @@ def tune(instances, levels, dev=None, reps=7, warmup=2, prof=None, src=None, cac
     with _lock:
         if not force:
             if key in _memo:
-                return dict(_memo[key], cached=True)
+                return dict(DEFAULT, **_memo[key], cached=True)
             if cache_path:
                 hit = _load(cache_path).get(key)
                 if hit:
                     _memo[key] = hit
-                    return dict(hit, cached=True)
+                    return dict(DEFAULT, **hit, cached=True)  # entries may predate a coordinate
         timings = {}
 
         def run(cand):
-            name = f"shoot_block={cand['shoot_block']},xpass_layout={cand['xpass_layout']}"
+            name = (f"shoot_block={cand['shoot_block']},xpass_layout={cand['xpass_layout']},"
+                f"fft_threads={cand['fft_threads']}")
             if name not in timings:
                 timings[name] = _time_plan(instances, levels, dev, cand, reps, warmup, prof, src)
             return timings[name]
@@ -121,6 +144,11 @@ def tune(instances, levels, dev=None, reps=7, warmup=2, prof=None, src=None, cac
                 best, best_ms = cand, ms
         for lay in XPASS_LAYOUTS:
             cand = dict(best, xpass_layout=lay)
+            ms = run(cand)
+            if ms < MARGIN * best_ms:
+                best, best_ms = cand, ms
+        for ft in fft_thread_candidates(instances):
+            cand = dict(best, fft_threads=ft)
             ms = run(cand)
             if ms < MARGIN * best_ms:
                 best, best_ms = cand, ms

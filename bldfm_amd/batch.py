@@ -140,7 +140,7 @@ class FootprintBatch:
             from . import autotune
 
             self.tune_report = autotune.tune(self.instances, ints, self.device)
-            tuning = {k: self.tune_report[k] for k in ("shoot_block", "xpass_layout")}
+            tuning = {k: self.tune_report[k] for k in ("shoot_block", "xpass_layout", "fft_threads")}
         self.plan = _lib.Plan(self.instances, ints, self.device, tuning=tuning)
         self.tuning = self.plan.tuning
         dev = self.device

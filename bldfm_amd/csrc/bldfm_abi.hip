@@ -522,6 +522,14 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     for (int e = 0; e < 2; ++e) V.variant_used[k][e] = variant_used[k][e];
   V.fft_small = max_n <= BLDFM_FFT_SMALL_N ? 1 : 0;
   V.fft_threads = fft_threads_for(max_n);
+  if (tuning && tuning->fft_threads) {
+    const int cap = V.fft_small ? 256 : 1024;  // launch bounds of the two FFT kernel families
+    const int t = tuning->fft_threads;
+    if (t % 64 != 0 || t < V.fft_threads || t > cap)
+      return bad(BLDFM_EINVAL, "fft_threads must be a multiple of 64 between " + std::to_string(V.fft_threads) +
+                                   " and " + std::to_string(cap) + " for this plan");
+    V.fft_threads = t;
+  }
   V.fft_lds_bytes = fft_lds_bytes(max_n);  // non-decreasing in n
   V.tot_tab = tiles[0][ninst];
   V.tot_fwd_rows = tiles[1][ninst];

@@ -147,12 +147,18 @@ int bldfm_plan_create(int device, const bldfm_instance* inst, int32_t ninst,
  *                 at stride ny; 1 the y-pass stores its columns transposed;
  *                 2 a coalesced LDS-tiled transpose runs between the passes
  *                 (one more y-pass-sized workspace block)
+ *   fft_threads   threads per workgroup of the FFT passes (one transform per
+ *                 workgroup).  0 selects ceil(n/8) (n <= 2048) or ceil(n/10),
+ *                 rounded up to a multiple of 64, for the plan's longest
+ *                 transform n; otherwise a multiple of 64 between that value
+ *                 and 256 (n <= 2048) or 1024.
  * Every choice gives the same results (bit for bit).
  */
 typedef struct {
   int32_t shoot_block;
   int32_t xpass_layout;
-  int32_t reserved[6];
+  int32_t fft_threads;
+  int32_t reserved[5];
 } bldfm_tuning;
 
 int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t ninst,

@@ -136,3 +136,21 @@ def test_autotune_shape_key():
     assert autotune.shape_key(a, cpu) == autotune.shape_key(list(reversed(a)), cpu)
     assert autotune.shape_key(a, cpu) != autotune.shape_key([inst(64, 20), inst(128, 31)], cpu)
     assert autotune.shape_key(a, cpu) != autotune.shape_key([inst(64, 20), inst(128, 30, 0)], cpu)
+
+
+def test_fft_thread_candidates():
+    """FFT workgroup sizes the autotuner times: the default (0) and larger
+    multiples of 64 up to the kernel family's bound (256 threads up to 2048
+    points, else 1024), from the longest padded transform of the batch."""
+    from bldfm_amd import _lib, autotune
+
+    def inst(n, pad):
+        i = _lib.Instance()
+        i.nx, i.ny, i.px, i.py = n, n, pad, pad
+        return i
+
+    assert autotune.fft_thread_candidates([inst(512, 512)]) == [0, 256]  # 1536 points: default 192
+    assert autotune.fft_thread_candidates([inst(128, 128)]) == [0, 128]  # 384 points: default 64
+    assert autotune.fft_thread_candidates([inst(256, 256), inst(128, 128)]) == [0, 192, 256]  # 768: 128
+    assert autotune.fft_thread_candidates([inst(2048, 0)]) == [0]  # 2048 points: 256 already
+    assert autotune.fft_thread_candidates([inst(1000, 1000)]) == [0, 384, 640]  # 3000 points: 320

@@ -1,7 +1,8 @@
 """Launch shapes (bldfm_tuning) and the autotuner.
 
-Every candidate shape -- modes per workgroup of the vertical solve and the
-three x-pass input layouts -- gives bit-identical results on a mixed batch
+Every candidate shape -- modes per workgroup of the vertical solve, the
+three x-pass input layouts and the FFT workgroup size -- gives bit-identical
+results on a mixed batch
 (several grids, vertical resolutions and meteorologies, one and several
 output levels): the shapes only regroup identical per-lane arithmetic.  The
 autotuner returns one of the candidates it timed, within its 1% switching
@@ -19,7 +20,10 @@ from test_gpu_batch import c3_like_specs
 
 pytestmark = pytest.mark.gpu
 
-CANDIDATES = [dict(shoot_block=b, xpass_layout=lay) for b in (64, 128, 256) for lay in (0, 1, 2)]
+CANDIDATES = [dict(shoot_block=b, xpass_layout=lay, fft_threads=0) for b in (64, 128, 256) for lay in (0, 1, 2)]
+# FFT workgroup sizes above the default (valid for any batch whose transforms fit the 256-thread kernels)
+CANDIDATES += [dict(shoot_block=256, xpass_layout=0, fft_threads=192), dict(shoot_block=256, xpass_layout=0, fft_threads=256),
+               dict(shoot_block=128, xpass_layout=2, fft_threads=256)]
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +49,8 @@ def reference(specs):
     return _outputs(specs, None)
 
 
-@pytest.mark.parametrize("tuning", CANDIDATES, ids=lambda t: f"b{t['shoot_block']}_x{t['xpass_layout']}")
+@pytest.mark.parametrize("tuning", CANDIDATES,
+                         ids=lambda t: f"b{t['shoot_block']}_x{t['xpass_layout']}_f{t['fft_threads']}")
 def test_every_shape_is_bitwise_identical(specs, reference, tuning):
     import torch
 
@@ -64,12 +69,16 @@ def test_autotune_picks_the_fastest_and_caches(specs, reference, tmp_path):
     r1 = autotune.tune(fb.instances, fb.levels_i32, fb.device, cache_path=path)
     assert not r1["cached"]
     assert r1["shoot_block"] in (64, 128, 256) and r1["xpass_layout"] in (0, 1, 2)
-    assert len(r1["ms"]) == 5  # coordinate search: default, two more blocks, two more layouts
-    chosen = f"shoot_block={r1['shoot_block']},xpass_layout={r1['xpass_layout']}"
+    fts = autotune.fft_thread_candidates(fb.instances)
+    assert fts[0] == 0 and len(fts) >= 2 and r1["fft_threads"] in fts
+    # coordinate search: default, two more blocks, two more layouts, the other FFT workgroup size
+    assert len(r1["ms"]) == 5 + len(fts) - 1
+    chosen = (f"shoot_block={r1['shoot_block']},xpass_layout={r1['xpass_layout']},"
+              f"fft_threads={r1['fft_threads']}")
     # the choice is within the 1% switching margin of the fastest candidate
     assert r1["ms"][chosen] == r1["best_ms"] <= min(r1["ms"].values()) / autotune.MARGIN
     r2 = autotune.tune(fb.instances, fb.levels_i32, fb.device, cache_path=path)
-    assert r2["cached"] and r2["shoot_block"] == r1["shoot_block"] and r2["xpass_layout"] == r1["xpass_layout"]
+    assert r2["cached"] and all(r2[k] == r1[k] for k in ("shoot_block", "xpass_layout", "fft_threads"))
     autotune.clear_memo()
     r3 = autotune.tune(fb.instances, fb.levels_i32, fb.device, cache_path=path)
     assert r3["cached"] and r3["best_ms"] == r1["best_ms"]
@@ -84,7 +93,8 @@ def test_auto_batch_matches_default(specs, reference):
     assert torch.equal(conc, reference[1]) and torch.equal(flx, reference[2])
 
 
-@pytest.mark.parametrize("bad", [dict(shoot_block=96), dict(xpass_layout=3), "fastest"])
+@pytest.mark.parametrize("bad", [dict(shoot_block=96), dict(xpass_layout=3), dict(fft_threads=100),
+                                 dict(fft_threads=1088), "fastest"])
 def test_invalid_shapes_are_refused(specs, bad):
     from bldfm_amd import _lib
     from bldfm_amd.batch import FootprintBatch
