@@ -154,3 +154,26 @@ def test_fft_thread_candidates():
     assert autotune.fft_thread_candidates([inst(256, 256), inst(128, 128)]) == [0, 192, 256]  # 768: 128
     assert autotune.fft_thread_candidates([inst(2048, 0)]) == [0]  # 2048 points: 256 already
     assert autotune.fft_thread_candidates([inst(1000, 1000)]) == [0, 384, 640]  # 3000 points: 320
+
+
+def test_autotune_cache_entries_get_new_coordinates(tmp_path):
+    """A cached choice written before a launch-shape coordinate existed comes
+    back with that coordinate at its default (no device work on a hit)."""
+    import json
+
+    import torch
+
+    from bldfm_amd import _lib, autotune
+
+    i = _lib.Instance()
+    i.nx, i.ny, i.nlx, i.nly, i.nz, i.nslot, i.footprint = 64, 64, 64, 64, 20, 1, 1
+    cpu = torch.device("cpu")
+    key = autotune.shape_key([i], cpu)
+    path = tmp_path / "tune.json"
+    path.write_text(json.dumps({key: {"shoot_block": 128, "xpass_layout": 1, "best_ms": 1.0, "ms": {}}}))
+    autotune.clear_memo()
+    r = autotune.tune([i], None, cpu, cache_path=str(path))
+    assert r["cached"] and r["shoot_block"] == 128 and r["xpass_layout"] == 1 and r["fft_threads"] == 0
+    r2 = autotune.tune([i], None, cpu, cache_path=str(path))  # now from memory
+    assert r2["cached"] and r2["fft_threads"] == 0
+    autotune.clear_memo()
