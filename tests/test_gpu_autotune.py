@@ -137,3 +137,31 @@ def test_layouts_on_dispersion_solves(precision):
     assert torch.isfinite(outs[0][0]).all() and float(outs[0][0].abs().max()) > 0
     for (c, f), t in zip(outs[1:], CANDIDATES):
         assert torch.equal(c, outs[0][0]) and torch.equal(f, outs[0][1]), t
+
+
+def test_fft_workgroup_sizes_of_the_large_transform_kernels():
+    """Transforms above 2048 points run in the 1024-thread kernel family;
+    every allowed workgroup size (the default ceil(n/10) rounded up to a wave,
+    the autotuner's candidates and the 1024-thread bound) gives the same
+    footprints bit for bit."""
+    import torch
+
+    from bldfm_amd import autotune
+    from bldfm_amd.batch import FootprintBatch, FootprintSpec
+
+    specs = [FootprintSpec(nx=1000, ny=1000, domain=(2000.0, 2000.0), modes=(256, 256), meas_pt=(1000.0, 1000.0),
+                           n=12, meas_height=10.0, wind=(3.0, 1.0), ustar=0.3, mol=-40.0)]
+    ref = FootprintBatch(specs, precision="double")
+    n = max(max(int(i.nx) + 2 * int(i.px), int(i.ny) + 2 * int(i.py)) for i in ref.instances)
+    assert n > 2048
+    cands = autotune.fft_thread_candidates(ref.instances)
+    assert len(cands) >= 2
+    ref.run()
+    torch.cuda.synchronize()
+    c0, f0 = ref.conc.clone(), ref.flx.clone()
+    assert torch.isfinite(c0).all() and float(f0.abs().max()) > 0
+    for ft in cands[1:] + [1024]:
+        fb = FootprintBatch(specs, precision="double", tuning=dict(shoot_block=256, xpass_layout=0, fft_threads=ft))
+        fb.run()
+        torch.cuda.synchronize()
+        assert torch.equal(fb.conc, c0) and torch.equal(fb.flx, f0), ft
