@@ -289,6 +289,19 @@ def test_fft2_length_limit(gpu):
         fft_manager.fft2(np.ones((2, 10240), dtype=complex))
 
 
+def test_solver_length_limit(gpu):
+    """A solve whose padded grid exceeds the single-workgroup FFT limit
+    (3400 cells with the default halo: 10200 points) is refused with the
+    same clear error, before any device buffer is allocated."""
+    from bldfm_amd import _lib
+
+    _, solver = gpu
+    z, profs = opbl.vertical_profiles(8, 10.0, (3.0, 1.0), ustar=0.3, mol=-40.0)
+    with pytest.raises((ValueError, _lib.BLDFMError), match="single-workgroup FFT limit"):
+        solver.steady_state_transport_solver(np.zeros((16, 3400)), z, profs, (3400.0, 16.0), 8, modes=(64, 16),
+                                             meas_pt=(1700.0, 8.0), footprint=True)
+
+
 def test_device_math_rounds_like_host(gpu):
     """Device hypot / complex sqrt / divide / multiply round exactly like the
     host's numpy on the value ranges the solver feeds them."""
