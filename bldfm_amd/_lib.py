@@ -9,6 +9,7 @@ import ctypes
 import threading
 from pathlib import Path
 
+import numpy as np
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "_bldfm_hip.so"
@@ -36,6 +37,11 @@ class Met(ctypes.Structure):
         ("domain_height", ctypes.c_double),
         ("stretch", ctypes.c_double),
         ("tke", ctypes.c_double),
+        ("has_prologue", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("grid_aa", ctypes.c_double),
+        ("grid_bb", ctypes.c_double),
+        ("grid_zetamax", ctypes.c_double),
     ]
 
 
@@ -163,6 +169,11 @@ def load():
         return lib
 
 
+def last_error():
+    """The library's last error message."""
+    return load().bldfm_last_error().decode(errors="replace")
+
+
 def check(rc, what="BLDFM call"):
     """Raise ValueError for argument errors, BLDFMError for the rest."""
     if rc == 0:
@@ -188,8 +199,15 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
-def make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height, stretch, tke):
-    """Fill a ``Met`` from vertical_profiles arguments (closure validated here)."""
+def make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height, stretch, tke, prologue=True):
+    """Fill a ``Met`` from vertical_profiles arguments (closure validated here).
+
+    With ``prologue`` (the default) the scalar prologue -- the derived z0 or
+    u* and the grid constants -- is evaluated here with numpy exactly as the
+    reference evaluates it (``bldfm_amd.prologue``), and the library only
+    evaluates the per-level profiles; otherwise the library derives them with
+    correctly rounded functions.
+    """
     if closure not in CLOSURES:
         raise ValueError(
             f"Invalid closure type: {closure}. "
@@ -212,7 +230,15 @@ def make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height,
     m.prsc = float(prsc)
     m.domain_height = float(domain_height) if domain_height is not None else 0.0
     m.stretch = float(stretch) if stretch is not None else 0.0
-    m.tke = float(tke) if tke is not None else 1.0
+    m.tke = float(np.squeeze(tke)) if tke is not None and np.size(tke) == 1 else 1.0
+    if prologue and (closure == "OAAHOC" or (ustar is None) != (z0 is None)):
+        from .prologue import prologue as _prologue
+
+        pz0, pus, aa, bb, zetamx = _prologue(meas_height, wind, ustar, z0, mol, closure, domain_height, stretch, tke)
+        m.has_prologue = 1
+        m.has_ustar = m.has_z0 = 1
+        m.z0, m.ustar = pz0, pus
+        m.grid_aa, m.grid_bb, m.grid_zetamax = aa, bb, zetamx
     return m
 
 

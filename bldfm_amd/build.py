@@ -50,15 +50,18 @@ def _run(cmd):
 
 
 def build(verbose: bool = True) -> Path:
+    """Compile every csrc/*.hip (in parallel: they are independent units) and link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     objdir = PKG / "_obj"
     objdir.mkdir(exist_ok=True)
-    objs = []
-    for src in sorted(CSRC.glob("*.hip")):
-        obj = objdir / (src.stem + ".o")
-        if verbose:
-            print(f"[bldfm build] hipcc {src.name}", flush=True)
-        _run([hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)])
-        objs.append(str(obj))
+    srcs = sorted(CSRC.glob("*.hip"))
+    objs = [str(objdir / (src.stem + ".o")) for src in srcs]
+    if verbose:
+        print(f"[bldfm build] hipcc {' '.join(s.name for s in srcs)}", flush=True)
+    with ThreadPoolExecutor(max_workers=len(srcs)) as pool:
+        for f in [pool.submit(_run, [hipcc(), *HIPCC_FLAGS, "-c", str(s), "-o", o]) for s, o in zip(srcs, objs)]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB)

@@ -234,12 +234,17 @@ const char* bldfm_last_error(void) { return g_err.c_str(); }
 
 static int met_setup(const bldfm_met* m, ProfSetup* s) {
   if (m->n <= 0) return fail(BLDFM_EINVAL, "n must be positive");
-  const int rc = prof_setup(m->n, m->meas_height, m->um, m->vm, m->has_ustar, m->ustar, m->has_z0,
-                            m->z0, m->mol, m->prsc, m->closure, m->has_domain_height,
-                            m->domain_height, m->has_stretch, m->stretch, m->tke, s);
+  const int rc =
+      m->has_prologue
+          ? prof_setup_given(m->n, m->meas_height, m->um, m->vm, m->ustar, m->z0, m->mol, m->prsc, m->closure,
+                             m->has_stretch, m->stretch, m->tke, m->grid_aa, m->grid_bb, m->grid_zetamax, s)
+          : prof_setup(m->n, m->meas_height, m->um, m->vm, m->has_ustar, m->ustar, m->has_z0, m->z0, m->mol,
+                       m->prsc, m->closure, m->has_domain_height, m->domain_height, m->has_stretch, m->stretch,
+                       m->tke, s);
   if (rc == -1) return fail(BLDFM_EINVAL, "Either z0 or ustar must be provided.");
   if (rc == -2) return fail(BLDFM_EINVAL, "Invalid closure type");
   if (rc == -3) return fail(BLDFM_EINVAL, "vertical grid is empty");
+  if (rc == -4) return fail(BLDFM_EINVAL, "arange: cannot compute length");
   if (!std::isfinite(s->z0) || !std::isfinite(s->ustar) || !std::isfinite(s->aa) ||
       !std::isfinite(s->bb) || s->nlev < 2)
     return fail(BLDFM_EINVAL, "profile parameters give a non-finite or degenerate grid");

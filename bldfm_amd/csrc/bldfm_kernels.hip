@@ -829,11 +829,7 @@ __global__ __launch_bounds__(256) void k_stability(const double* __restrict__ x,
 __global__ __launch_bounds__(256) void k_wind(const double* __restrict__ spd, const double* __restrict__ dir,
                                               int64_t n, double* __restrict__ u, double* __restrict__ v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double th = dir[i] * (BLDFM_PI / 180.0);
-    double s, c;
-    sincos(th, &s, &c);
-    u[i] = -spd[i] * s;
-    v[i] = -spd[i] * c;
+    wind_uv(spd[i], dir[i], &u[i], &v[i]);
   }
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "bldfm_types.h"
+#include "bldfm_crmath.h"
 
 // ------------------------------------------------------------------ complex
 BLDFM_HD double2 cmk(double re, double im) { double2 r; r.x = re; r.y = im; return r; }
@@ -156,21 +157,38 @@ enum Closure : int32_t { CL_MOST = 0, CL_CONSTANT = 1, CL_MOSTM = 2, CL_OAAHOC =
 #define BLDFM_KARMAN 0.4
 #define BLDFM_PI 3.141592653589793
 
-// Positive real power written as exp(p ln v): the value numpy's complex
-// power returns for a positive real base.
-BLDFM_HD double real_cpow(double v, double p) { return exp(p * log(v)); }
+// The profile formulas below round as the reference's numpy/glibc evaluation
+// does: every product and sum on its own, in the reference's order (no
+// contraction), with the correctly rounded or glibc-compatible functions of
+// bldfm_crmath.h in place of the device libm.
 
-// MOST momentum stability correction psi(z/L) (src/bldfm/pbl_model.py:207-230).
+// MOST momentum stability correction psi(z/L) (src/bldfm/pbl_model.py:207-230):
+// 5x for x > 0, else with xi = Re((1 - 16x)^(1/4)) (complex power)
+// -2 ln((1 + xi)/2) - ln((1 + xi^2)/2) + 2 atan(xi) - pi/2.
 BLDFM_HD double most_psi(double x) {
+  CR_EXACT
   if (x > 0.0) return 5.0 * x;
-  const double xi = real_cpow(1.0 - 16.0 * x, 0.25);
-  return -2.0 * log(0.5 * (1.0 + xi)) - log(0.5 * (1.0 + xi * xi)) + 2.0 * atan(xi) - 0.5 * BLDFM_PI;
+  const double xi = gl_cpow_real(1.0 - 16.0 * x, 0.25);
+  return -2.0 * cr_log(0.5 * (1.0 + xi)) - cr_log(0.5 * (1.0 + xi * xi)) + 2.0 * cr_atan(xi) - 0.5 * BLDFM_PI;
 }
 
 // MOST diffusivity stability function phi(z/L) (src/bldfm/pbl_model.py:233-250).
 BLDFM_HD double most_phi(double x) {
+  CR_EXACT
   if (x > 0.0) return 1.0 + 5.0 * x;
-  return real_cpow(1.0 - 16.0 * x, -0.5);
+  return gl_cpow_real(1.0 - 16.0 * x, -0.5);
+}
+
+// Met wind (speed, direction blowing FROM, degrees) -> (u, v)
+// (src/bldfm/utils.py:23-27): theta = dir * (pi/180) as numpy's deg2rad
+// rounds it, u = -U sin(theta), v = -U cos(theta).
+BLDFM_HD void wind_uv(double spd, double dir, double* u, double* v) {
+  CR_EXACT
+  const double th = dir * (BLDFM_PI / 180.0);
+  double s, c;
+  cr_sincos(th, &s, &c);
+  *u = -spd * s;
+  *v = -spd * c;
 }
 
 // Scalar prologue of vertical_profiles: derived roughness or friction
@@ -187,21 +205,22 @@ BLDFM_HD int prof_setup(int n, double zm, double um, double vm, int has_ustar, d
                         int has_z0, double z0, double mol, double prsc, int closure,
                         int has_domain_height, double domain_height, int has_stretch,
                         double stretch, double tke, ProfSetup* s) {
+  CR_EXACT
   const double kap = BLDFM_KARMAN;
   s->zm = zm; s->um = um; s->vm = vm; s->mol = mol; s->prsc = prsc; s->closure = closure;
   s->tke = tke;
   s->absum = sqrt(um * um + vm * vm);
   if (closure == CL_MOST || closure == CL_CONSTANT || closure == CL_MOSTM) {
     if (!has_z0 && has_ustar) {
-      z0 = zm * exp(-kap * s->absum / ustar + most_psi(zm / mol));
+      z0 = zm * cr_exp(-kap * s->absum / ustar + most_psi(zm / mol));
     } else if (!has_ustar && has_z0) {
-      ustar = s->absum * kap / (log(zm / z0) + most_psi(zm / mol));
+      ustar = s->absum * kap / (cr_log(zm / z0) + most_psi(zm / mol));
     } else {
       return -1;
     }
   } else if (closure == CL_OAAHOC) {
     const double cl = 0.845, cm = 0.0856;
-    z0 = zm * exp(-cm * cl * s->absum * sqrt(tke) / (ustar * ustar));
+    z0 = zm * cr_exp(-cm * cl * s->absum * sqrt(tke) / (ustar * ustar));
   } else {
     return -2;
   }
@@ -209,9 +228,9 @@ BLDFM_HD int prof_setup(int n, double zm, double um, double vm, int has_ustar, d
   s->z0 = z0;
   s->h = has_stretch ? stretch : 2.0 * zm;
   const double zmx = has_domain_height ? domain_height : 2.0 * zm;
-  s->bb = zm / (exp(-z0 / s->h) - exp(-zm / s->h));
-  s->aa = s->bb * exp(-z0 / s->h);
-  const double zetamx = s->aa - s->bb * exp(-zmx / s->h);
+  s->bb = zm / (cr_exp(-z0 / s->h) - cr_exp(-zm / s->h));
+  s->aa = s->bb * cr_exp(-z0 / s->h);
+  const double zetamx = s->aa - s->bb * cr_exp(-zmx / s->h);
   s->dzeta = zm / n;
   // numpy arange length: ceil((stop - start) / step)
   const double len = ceil((zetamx + s->dzeta) / s->dzeta);
@@ -220,18 +239,45 @@ BLDFM_HD int prof_setup(int n, double zm, double um, double vm, int has_ustar, d
   return 0;
 }
 
+// The same prologue from a caller's values of the derived z0 / u* and the
+// grid constants aa, bb, zeta_max (the reference's own numpy evaluation of
+// src/bldfm/pbl_model.py:65-120); h, dzeta and the numpy arange length
+// ceil((zeta_max + dzeta)/dzeta) are formed here.  Returns 0, -2 for an
+// invalid closure, -3 for an empty grid, -4 when the length is not finite.
+BLDFM_HD int prof_setup_given(int n, double zm, double um, double vm, double ustar, double z0, double mol,
+                              double prsc, int closure, int has_stretch, double stretch, double tke,
+                              double aa, double bb, double zetamx, ProfSetup* s) {
+  CR_EXACT
+  if (closure != CL_MOST && closure != CL_CONSTANT && closure != CL_MOSTM && closure != CL_OAAHOC) return -2;
+  s->zm = zm; s->um = um; s->vm = vm; s->mol = mol; s->prsc = prsc; s->closure = closure;
+  s->tke = tke;
+  s->absum = sqrt(um * um + vm * vm);
+  s->ustar = ustar;
+  s->z0 = z0;
+  s->h = has_stretch ? stretch : 2.0 * zm;
+  s->aa = aa;
+  s->bb = bb;
+  s->dzeta = zm / n;
+  const double len = ceil((zetamx + s->dzeta) / s->dzeta);
+  if (!(len == len) || len > 1e9) return -4;
+  if (!(len >= 1.0)) return -3;
+  s->nlev = (int32_t)len;
+  return 0;
+}
+
 // Height and profile values at grid point i (src/bldfm/pbl_model.py:127-182).
 // out = {z, u, v, Kx, Ky, Kz}
 BLDFM_HD void prof_point(const ProfSetup& s, int i, double out[6]) {
+  CR_EXACT
   const double kap = BLDFM_KARMAN;
   const double zeta = 0.0 + i * s.dzeta;
-  const double z = -s.h * log(-(zeta - s.aa) / s.bb);
+  const double z = -s.h * cr_log(-(zeta - s.aa) / s.bb);
   double u, v, kx, ky, kz;
   if (s.closure == CL_CONSTANT) {
     const double km = kap * s.ustar * s.zm / s.prsc;
     u = s.um; v = s.vm; kx = ky = kz = km;
   } else if (s.closure == CL_MOST || s.closure == CL_MOSTM) {
-    const double absu = s.ustar / kap * (log(z / s.z0) + most_psi(z / s.mol));
+    const double absu = s.ustar / kap * (cr_log(z / s.z0) + most_psi(z / s.mol));
     u = s.um / s.absum * absu;
     v = s.vm / s.absum * absu;
     const double K = kap * s.ustar * z / most_phi(z / s.mol) / s.prsc;
@@ -244,7 +290,7 @@ BLDFM_HD void prof_point(const ProfSetup& s, int i, double out[6]) {
     }
   } else {  // OAAHOC
     const double cl = 0.845, cm = 0.0856, ch = 0.204;
-    const double absu = s.ustar * s.ustar / cm / cl / sqrt(s.tke) * log(z / s.z0);
+    const double absu = s.ustar * s.ustar / cm / cl / sqrt(s.tke) * cr_log(z / s.z0);
     u = s.um / s.absum * absu;
     v = s.vm / s.absum * absu;
     kx = ky = kz = ch * cl * z * sqrt(s.tke);

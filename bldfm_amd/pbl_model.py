@@ -56,14 +56,17 @@ def vertical_profiles(
             tke = 1.0
         if ustar is None:
             raise TypeError("the OAAHOC closure needs ustar")
-        if np.ndim(tke) != 0 and np.size(tke) != 1:
-            raise ValueError("the OAAHOC closure supports a scalar tke")
-        tke = float(np.squeeze(tke))
+    # the scalar prologue runs here in numpy (bldfm_amd.prologue); it raises
+    # the reference's ValueError for an OAAHOC tke array of several values
     met = _lib.make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height, stretch, tke)
     lib = _lib.load()
     dev = _lib.device()
     nlev, z0o, ustaro = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
-    _lib.check(lib.bldfm_profile_query(ctypes.byref(met), ctypes.byref(nlev), ctypes.byref(z0o), ctypes.byref(ustaro)))
+    rc = lib.bldfm_profile_query(ctypes.byref(met), ctypes.byref(nlev), ctypes.byref(z0o), ctypes.byref(ustaro))
+    if rc != 0 and "grid is empty" in _lib.last_error():
+        # the reference's stats logging calls max() on the empty profiles
+        raise ValueError("max() arg is an empty sequence")
+    _lib.check(rc)
     out = torch.empty(6 * nlev.value, dtype=torch.float64, device=dev)
     _lib.check(lib.bldfm_profiles(ctypes.byref(met), _lib.ptr(out), _lib.stream_handle(dev)), "profiles")
     z, u, v, kx, ky, kz = out.cpu().numpy().reshape(6, nlev.value)
@@ -72,9 +75,14 @@ def vertical_profiles(
         Kx, Ky, Kz = kx.copy(), ky.copy(), kz.copy()
     else:
         Kx = Ky = Kz = kz.copy()
+    if closure == "OAAHOC" and np.ndim(tke) > 0:
+        # tke[..., newaxis] broadcasts the grid and profiles to tke.shape + (nlev,)
+        shape = np.shape(tke) + (nlev.value,)
+        z, u, v = z.reshape(shape), u.reshape(shape), v.reshape(shape)
+        Kx = Ky = Kz = Kx.reshape(shape)
 
     logger.info("Stats from vertical_profiles")
-    logger.info("z0    = %.3f m", z[0])
+    logger.info("z0    = %.3f m", float(np.ravel(z)[0]))
     logger.info("ustar = %.3f m s-1", ustaro.value)
     return z, (u, v, Kx, Ky, Kz)
 

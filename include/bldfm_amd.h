@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define BLDFM_ABI_VERSION 1
+#define BLDFM_ABI_VERSION 2
 
 /* Error codes */
 #define BLDFM_OK 0
@@ -47,7 +47,18 @@ extern "C" {
  * Meteorology / profile parameters: the arguments of
  * vertical_profiles(n, meas_height, wind, ustar, z0, mol, prsc, closure,
  * domain_height, stretch, z0_min, z0_max, tke)   src/bldfm/pbl_model.py:8-22.
- * Exactly one of has_ustar/has_z0 for MOST, MOSTM and CONSTANT.
+ * Exactly one of has_ustar/has_z0 for MOST, MOSTM and CONSTANT, unless
+ * has_prologue is set.
+ *
+ * The scalar prologue (src/bldfm/pbl_model.py:65-120: the derived z0 or u*,
+ * the stretched-grid constants aa = bb e^{-z0/h}, bb = zm/(e^{-z0/h} - e^{-zm/h})
+ * and zeta_max = aa - bb e^{-zmx/h}) is a handful of scalar exp/log calls.
+ * When has_prologue is 0 the library evaluates it with correctly rounded
+ * functions.  A caller that must reproduce a particular host's libm bit for
+ * bit (the Python drop-in passes numpy's own values: numpy's exp is not
+ * correctly rounded on AVX-512 hosts) sets has_prologue = 1 and both of
+ * ustar and z0 plus grid_aa, grid_bb and grid_zetamax; the library then
+ * uses them as given and evaluates only the per-level profiles.
  */
 typedef struct bldfm_met {
   int32_t n;                 /* grid points between z0 and meas_height */
@@ -56,6 +67,9 @@ typedef struct bldfm_met {
   double meas_height;        /* zm [m] */
   double um, vm;             /* wind components at zm [m/s] */
   double ustar, z0, mol, prsc, domain_height, stretch, tke;
+  int32_t has_prologue;      /* 1: ustar, z0 and grid_* below are the caller's prologue */
+  int32_t reserved;
+  double grid_aa, grid_bb, grid_zetamax;
 } bldfm_met;
 
 /*

@@ -7,6 +7,9 @@ import numpy as np
 
 KARMAN = 0.4  # von Karman constant, src/bldfm/pbl_model.py:60
 
+# planted mistakes of vertical_profiles, each of which a pinning test must catch
+VALID_MUTATIONS = (None, "mostm_swap", "oaahoc_cm", "oaahoc_k", "stretch_ignored")
+
 
 def compute_wind_fields(u_rot, wind_dir):
     """Met direction (deg, blowing FROM, clockwise from north) -> (u, v).
@@ -68,7 +71,7 @@ def phi(x):
     )
 
 
-def stretched_grid(n, zm, z0, domain_height=None, stretch=None):
+def stretched_grid(n, zm, z0, domain_height=None, stretch=None, _mutation=None):
     """Log-stretched vertical grid, src/bldfm/pbl_model.py:105-129.
 
     zeta is uniform with step zm/n from 0 past zeta(zmx); the mapping
@@ -76,7 +79,7 @@ def stretched_grid(n, zm, z0, domain_height=None, stretch=None):
     length is numpy ``arange``'s ceil((stop - start)/step) and the last point
     may overshoot ``domain_height``.
     """
-    h = 2.0 * zm if stretch is None else stretch
+    h = 2.0 * zm if stretch is None or _mutation == "stretch_ignored" else stretch
     zmx = 2.0 * zm if domain_height is None else domain_height
     bb = zm / (np.exp(-z0 / h) - np.exp(-zm / h))
     aa = bb * np.exp(-z0 / h)
@@ -100,13 +103,17 @@ def vertical_profiles(
     z0_min=0.001,
     z0_max=2.0,
     tke=None,
+    _mutation=None,
 ):
     """Grid and profiles (u, v, Kx, Ky, Kz); src/bldfm/pbl_model.py:8-204.
 
     Exactly one of ``ustar``/``z0`` for the MOST-type closures: the other is
     derived from |U(zm)| by the log law (src/bldfm/pbl_model.py:65-77).
     ``z0_min``/``z0_max`` are accepted and unused, as in the reference.
+    ``_mutation`` plants a known mistake for the tests that pin this oracle
+    (one of ``VALID_MUTATIONS``); None is the restatement itself.
     """
+    assert _mutation in VALID_MUTATIONS, _mutation
     zm = meas_height
     um, vm = wind
     absum = np.sqrt(um**2 + vm**2)
@@ -129,8 +136,12 @@ def vertical_profiles(
             "Supported closures are 'MOST', 'CONSTANT', and 'OAAHOC'."
         )
 
-    z = stretched_grid(n, zm, z0, domain_height, stretch)
-    nlev = len(z)
+    z = stretched_grid(n, zm, z0, domain_height, stretch, _mutation)
+    nlev = z.shape[-1]
+    if nlev == 0:
+        # the reference's stats logging evaluates max(u) on the empty profiles
+        # (src/bldfm/pbl_model.py:184-194)
+        raise ValueError("max() arg is an empty sequence")
 
     if closure == "CONSTANT":
         K = KARMAN * ustar * zm / prsc * np.ones(nlev)
@@ -142,9 +153,15 @@ def vertical_profiles(
         K = KARMAN * ustar * z / phi(z / mol) / prsc
         if closure == "MOST":
             return z, (u, v, K, K, K)
+        if _mutation == "mostm_swap":  # planted: along- and cross-wind swapped
+            return z, (u, v, K * u**2 / (u**2 + v**2), K * v**2 / (u**2 + v**2), K)
         return z, (u, v, K * v**2 / (u**2 + v**2), K * u**2 / (u**2 + v**2), K)
     # OAAHOC (Schumann-Lilly one-and-a-half order closure)
     cl, cm, ch = 0.845, 0.0856, 0.204
+    if _mutation == "oaahoc_cm":  # planted: the wind law uses c_h for c_m
+        cm = ch
     absu = ustar**2 / cm / cl / np.sqrt(tke) * np.log(z / z0)
     K = ch * cl * z * np.sqrt(tke)
+    if _mutation == "oaahoc_k":  # planted: diffusivity without sqrt(tke)
+        K = ch * cl * z * tke
     return z, (um / absum * absu, vm / absum * absu, K, K, K)

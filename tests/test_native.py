@@ -44,7 +44,7 @@ def test_header_symbols_exported_and_bound(lib):
     for name in names:
         assert hasattr(lib, name), name
         assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.bldfm_abi_version() == 1
+    assert lib.bldfm_abi_version() == 2
 
 
 def test_exports_are_exactly_the_header(lib):
@@ -93,12 +93,15 @@ def test_profile_query_matches_oracle(lib, kw):
 
     args = dict(ustar=None, z0=None, mol=1e9, prsc=1.0, closure="MOST", domain_height=None, stretch=None, tke=None)
     args.update({k: v for k, v in kw.items() if k not in ("n", "meas_height", "wind")})
-    met = _lib.make_met(kw["n"], kw["meas_height"], kw["wind"], **args)
-    n, z0, us = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
-    _lib.check(lib.bldfm_profile_query(ctypes.byref(met), ctypes.byref(n), ctypes.byref(z0), ctypes.byref(us)))
     z, _ = opbl.vertical_profiles(**kw)
-    assert n.value == len(z)
-    np.testing.assert_allclose(z0.value, z[0], rtol=1e-12)
+    # the host's numpy prologue (the drop-in's default) and the library's own
+    for prologue in (True, False):
+        met = _lib.make_met(kw["n"], kw["meas_height"], kw["wind"], prologue=prologue, **args)
+        assert met.has_prologue == prologue
+        n, z0, us = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        _lib.check(lib.bldfm_profile_query(ctypes.byref(met), ctypes.byref(n), ctypes.byref(z0), ctypes.byref(us)))
+        assert n.value == len(z)
+        np.testing.assert_allclose(z0.value, z[0], rtol=1e-12)
 
 
 def test_profile_query_errors(lib):
