@@ -111,6 +111,8 @@ _D = ctypes.c_double
 SIGNATURES = {
     "bldfm_abi_version": (ctypes.c_int, []),
     "bldfm_last_error": (ctypes.c_char_p, []),
+    "bldfm_mem_info": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                      ctypes.POINTER(ctypes.c_size_t)]),
     "bldfm_profile_query": (ctypes.c_int, [ctypes.POINTER(Met), ctypes.POINTER(_I32), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "bldfm_profiles": (ctypes.c_int, [ctypes.POINTER(Met), _P, _P]),
     "bldfm_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Instance), _I32, ctypes.POINTER(_I32), ctypes.POINTER(_P)]),
@@ -167,6 +169,14 @@ def load():
             fn.argtypes = args
         _lib = lib
         return lib
+
+
+def mem_info(device=None):
+    """(bytes held by the library's caches, free, total) on the device."""
+    dev = device if device is not None else torch.cuda.current_device()
+    c, f, t = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+    check(load().bldfm_mem_info(int(dev), ctypes.byref(c), ctypes.byref(f), ctypes.byref(t)), "mem_info")
+    return c.value, f.value, t.value
 
 
 def last_error():
@@ -240,6 +250,40 @@ def make_met(n, meas_height, wind, ustar, z0, mol, prsc, closure, domain_height,
         m.z0, m.ustar = pz0, pus
         m.grid_aa, m.grid_bb, m.grid_zetamax = aa, bb, zetamx
     return m
+
+
+def make_mets(specs):
+    """``make_met`` for a list of specs (objects with the vertical_profiles
+    fields n, meas_height, wind, ustar, z0, mol, prsc, closure,
+    domain_height, stretch, tke).  The numpy prologue of MOST-type instances
+    that share which of u*/z0 is given and whether domain_height/stretch are
+    set is evaluated in one vectorised pass (same values as one at a time)."""
+    from .prologue import prologue_many
+
+    mets = [make_met(s.n, s.meas_height, s.wind, s.ustar, s.z0, s.mol, s.prsc, s.closure, s.domain_height,
+                     s.stretch, s.tke, prologue=False) for s in specs]
+    groups = {}
+    for i, s in enumerate(specs):
+        if s.closure in ("CONSTANT", "MOST", "MOSTM") and (s.ustar is None) != (s.z0 is None):
+            key = (s.closure, s.ustar is None, s.domain_height is None, s.stretch is None)
+            groups.setdefault(key, []).append(i)
+        else:
+            mets[i] = make_met(s.n, s.meas_height, s.wind, s.ustar, s.z0, s.mol, s.prsc, s.closure,
+                               s.domain_height, s.stretch, s.tke)
+    for (closure, z0_given, no_dh, no_st), idx in groups.items():
+        sp = [specs[i] for i in idx]
+        col = lambda f: [getattr(s, f) for s in sp]  # noqa: E731
+        z0, us, aa, bb, zmx = prologue_many(
+            col("meas_height"), [s.wind[0] for s in sp], [s.wind[1] for s in sp],
+            None if z0_given else col("ustar"), col("z0") if z0_given else None, col("mol"), closure,
+            None if no_dh else col("domain_height"), None if no_st else col("stretch"))
+        for j, i in enumerate(idx):
+            m = mets[i]
+            m.has_prologue = 1
+            m.has_ustar = m.has_z0 = 1
+            m.z0, m.ustar = float(z0[j]), float(us[j])
+            m.grid_aa, m.grid_bb, m.grid_zetamax = float(aa[j]), float(bb[j]), float(zmx[j])
+    return mets
 
 
 class Tuning(ctypes.Structure):

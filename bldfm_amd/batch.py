@@ -56,7 +56,7 @@ class FootprintSpec:
     extra: dict = field(default_factory=dict)
 
 
-def _instance(spec: FootprintSpec, precision: str, fast: bool, surrogate: bool = False):
+def _instance(spec: FootprintSpec, precision: str, fast: bool, surrogate: bool = False, met=None):
     xmx, ymx = spec.domain
     nlx, nly = spec.modes
     if (nlx % 2 > 0) or (nly % 2 > 0):
@@ -70,10 +70,11 @@ def _instance(spec: FootprintSpec, precision: str, fast: bool, surrogate: bool =
     nxe, nye = nx + 2 * px, ny + 2 * py
     if nlx > nxe or nly > nye:
         nlx, nly = nxe, nye
-    met = _lib.make_met(spec.n, spec.meas_height, spec.wind, spec.ustar, spec.z0, spec.mol, spec.prsc,
-                        spec.closure, spec.domain_height, spec.stretch, spec.tke)
     if spec.closure in ("CONSTANT", "MOST", "MOSTM") and spec.ustar is not None and spec.z0 is not None:
         raise ValueError("Either z0 or ustar must be provided.")
+    if met is None:
+        met = _lib.make_met(spec.n, spec.meas_height, spec.wind, spec.ustar, spec.z0, spec.mol, spec.prsc,
+                            spec.closure, spec.domain_height, spec.stretch, spec.tke)
     lib = _lib.load()
     nlev = ctypes.c_int32()
     _lib.check(lib.bldfm_profile_query(ctypes.byref(met), ctypes.byref(nlev), None, None), "profile_query")
@@ -129,7 +130,11 @@ class FootprintBatch:
             raise ValueError("empty batch")
         self.specs = list(specs)
         self.device = _lib.device()
-        built = [_instance(s, precision, fast, surrogate is not None) for s in self.specs]
+        for s in self.specs:
+            if s.closure in ("CONSTANT", "MOST", "MOSTM") and s.ustar is not None and s.z0 is not None:
+                raise ValueError("Either z0 or ustar must be provided.")
+        mets = _lib.make_mets(self.specs)
+        built = [_instance(s, precision, fast, surrogate is not None, m) for s, m in zip(self.specs, mets)]
         self.instances = [b[0] for b in built]
         ints = np.concatenate([b[1] for b in built]).astype(np.int32)
         self.levels_i32 = ints
@@ -173,8 +178,7 @@ class FootprintBatch:
                 raise ValueError(f"instance {i}: update may not change n (the output level)")
             if new.closure in ("CONSTANT", "MOST", "MOSTM") and new.ustar is not None and new.z0 is not None:
                 raise ValueError("Either z0 or ustar must be provided.")
-        mets = [_lib.make_met(s.n, s.meas_height, s.wind, s.ustar, s.z0, s.mol, s.prsc, s.closure, s.domain_height,
-                              s.stretch, s.tke) for s in specs]
+        mets = _lib.make_mets(specs)
         self.plan.update_met(mets)
         for inst, m in zip(self.instances, mets):
             inst.met = m

@@ -131,11 +131,49 @@ int check_fft_len(int n, int device, const char* what) {
 }
 
 // Plans of the plain 2-D FFT, per (device, n): one device block holding the
-// FftPlan followed by its n twiddles.
+// FftPlan followed by its n twiddles.  Least recently used blocks are freed
+// once a device's blocks exceed kTwCacheBytes; the device is synchronised
+// first, so no kernel still in flight can read a freed block.
+struct TwBlock {
+  char* ptr;
+  size_t bytes;
+  uint64_t last_use;
+};
 std::mutex g_tw_mu;
-std::map<std::pair<int, int>, char*> g_tw_cache;
-constexpr size_t kPlanHdr = 256;  // FftPlan padded to 256 bytes
+std::map<std::pair<int, int>, TwBlock> g_tw_cache;
+uint64_t g_tw_clock = 0;
+constexpr size_t kPlanHdr = 256;                // FftPlan padded to 256 bytes
+constexpr size_t kTwCacheBytes = 16u << 20;     // per device
 static_assert(sizeof(FftPlan) <= kPlanHdr, "FftPlan must fit its header");
+
+size_t tw_cache_bytes(int device) {
+  size_t b = 0;
+  for (const auto& kv : g_tw_cache)
+    if (kv.first.first == device) b += kv.second.bytes;
+  return b;
+}
+
+// Frees least recently used blocks of `device` (except `keep`) until `need`
+// more bytes fit under the cap.  Called with g_tw_mu held and the device current.
+void tw_evict(int device, size_t need, std::pair<int, int> keep) {
+  size_t used = tw_cache_bytes(device);
+  bool synced = false;
+  while (used + need > kTwCacheBytes) {
+    auto victim = g_tw_cache.end();
+    for (auto it = g_tw_cache.begin(); it != g_tw_cache.end(); ++it)
+      if (it->first.first == device && it->first != keep &&
+          (victim == g_tw_cache.end() || it->second.last_use < victim->second.last_use))
+        victim = it;
+    if (victim == g_tw_cache.end()) break;
+    if (!synced) {
+      (void)hipDeviceSynchronize();
+      synced = true;
+    }
+    (void)hipFree(victim->second.ptr);
+    used -= victim->second.bytes;
+    g_tw_cache.erase(victim);
+  }
+}
 
 int cached_plan(int device, int n, const FftPlan** plan, const double2** tw) {
   std::lock_guard<std::mutex> lk(g_tw_mu);
@@ -148,15 +186,17 @@ int cached_plan(int device, int n, const FftPlan** plan, const double2** tw) {
     std::vector<char> h(kPlanHdr + sizeof(double2) * n, 0);
     memcpy(h.data(), &F, sizeof(F));
     twiddles(n, (double2*)(h.data() + kPlanHdr));
+    tw_evict(device, h.size(), key);
     char* d = nullptr;
     hipError_t e = hipMalloc((void**)&d, h.size());
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(fft plan)");
     e = hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) { (void)hipFree(d); return hip_fail(e, "hipMemcpy(fft plan)"); }
-    it = g_tw_cache.emplace(key, d).first;
+    it = g_tw_cache.emplace(key, TwBlock{d, h.size(), 0}).first;
   }
-  *plan = (const FftPlan*)it->second;
-  *tw = (const double2*)(it->second + kPlanHdr);
+  it->second.last_use = ++g_tw_clock;
+  *plan = (const FftPlan*)it->second.ptr;
+  *tw = (const double2*)(it->second.ptr + kPlanHdr);
   return 0;
 }
 
@@ -211,8 +251,8 @@ static SurMet sur_met_of(const ProfSetup& s, int n) {
   m.nw = (float)nw;
   double a[6], b[6];
   for (int j = 0; j < nw; ++j) {
-    prof_point(s, j, a);
-    prof_point(s, j + 1, b);
+    prof_point<false>(s, j, a);  // float features: libm suffices
+    prof_point<false>(s, j + 1, b);
     const double ua = a[1] * wx + a[2] * wy, ub = b[1] * wx + b[2] * wy;
     m.wkb[j][0] = (float)(b[0] - a[0]);
     m.wkb[j][1] = (float)((ua + ub) / (a[5] + b[5]));
@@ -231,6 +271,23 @@ extern "C" {
 int bldfm_abi_version(void) { return BLDFM_ABI_VERSION; }
 
 const char* bldfm_last_error(void) { return g_err.c_str(); }
+
+int bldfm_mem_info(int device, size_t* cache_bytes, size_t* free_bytes, size_t* total_bytes) {
+  DeviceGuard guard(device);
+  if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+  if (cache_bytes) {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    *cache_bytes = tw_cache_bytes(device);
+  }
+  if (free_bytes || total_bytes) {
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    if (e != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+  }
+  return 0;
+}
 
 static int met_setup(const bldfm_met* m, ProfSetup* s) {
   if (m->n <= 0) return fail(BLDFM_EINVAL, "n must be positive");
@@ -316,6 +373,7 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
   int max_n = 1;
 
   auto bad = [&](int code, const std::string& m) {
+    if (P->dmeta) (void)hipFree(P->dmeta);
     delete P;
     return fail(code, m);
   };
@@ -447,6 +505,18 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     off_ws += (ninst + 3) / 4;
   }
 
+  // launch shape of the FFT passes (validated before any device allocation)
+  const int fft_small = max_n <= BLDFM_FFT_SMALL_N ? 1 : 0;
+  int fft_threads = fft_threads_for(max_n);
+  if (tuning && tuning->fft_threads) {
+    const int cap = fft_small ? 256 : 1024;  // launch bounds of the two FFT kernel families
+    const int t = tuning->fft_threads;
+    if (t % 64 != 0 || t < fft_threads || t > cap)
+      return bad(BLDFM_EINVAL, "fft_threads must be a multiple of 64 between " + std::to_string(fft_threads) +
+                                   " and " + std::to_string(cap) + " for this plan");
+    fft_threads = t;
+  }
+
   // FFT plans and twiddles
   std::vector<FftPlan> plans(plan_n.size());
   int64_t tw_total = 0;
@@ -489,7 +559,7 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
   for (int k = 0; k < kKinds; ++k)
     if (!recs[k].empty()) memcpy(blob.data() + o_tiles[k], recs[k].data(), sizeof(TileRec) * recs[k].size());
   hipError_t e = hipMalloc(&P->dmeta, off);
-  if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(plan metadata)"); }
+  if (e != hipSuccess) { P->dmeta = nullptr; delete P; return hip_fail(e, "hipMalloc(plan metadata)"); }
   e = hipMemcpy(P->dmeta, blob.data(), off, hipMemcpyHostToDevice);
   if (e != hipSuccess) { (void)hipFree(P->dmeta); delete P; return hip_fail(e, "hipMemcpy(plan metadata)"); }
 
@@ -525,16 +595,8 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
   V.any_dispersion = any_disp;
   for (int k = 0; k < 4; ++k)
     for (int e = 0; e < 2; ++e) V.variant_used[k][e] = variant_used[k][e];
-  V.fft_small = max_n <= BLDFM_FFT_SMALL_N ? 1 : 0;
-  V.fft_threads = fft_threads_for(max_n);
-  if (tuning && tuning->fft_threads) {
-    const int cap = V.fft_small ? 256 : 1024;  // launch bounds of the two FFT kernel families
-    const int t = tuning->fft_threads;
-    if (t % 64 != 0 || t < V.fft_threads || t > cap)
-      return bad(BLDFM_EINVAL, "fft_threads must be a multiple of 64 between " + std::to_string(V.fft_threads) +
-                                   " and " + std::to_string(cap) + " for this plan");
-    V.fft_threads = t;
-  }
+  V.fft_small = fft_small;
+  V.fft_threads = fft_threads;
   V.fft_lds_bytes = fft_lds_bytes(max_n);  // non-decreasing in n
   V.tot_tab = tiles[0][ninst];
   V.tot_fwd_rows = tiles[1][ninst];
@@ -843,7 +905,9 @@ static bool sur_in_hull(const bldfm_plan* P, int i, const bldfm_sur_hull& H) {
   if (!(s.z0 >= H.z0_lo && s.z0 <= H.z0_hi && s.z0 <= H.z0_zm_max * s.zm)) return false;
   if (!(s.zm >= H.zm_lo && s.zm <= H.zm_hi)) return false;
   const double kx = D.sx * (D.nlx / 2) * D.vx, ky = D.sy * (D.nly / 2) * D.vy;
-  return kx <= H.kmax && ky <= H.kmax;
+  if (!(kx <= H.kmax && ky <= H.kmax)) return false;
+  // enough spot-checked modes for the gate's error estimate to mean something
+  return D.lanes / BLDFM_SUR_SPOT_STRIDE >= BLDFM_SUR_MIN_SPOT;
 }
 
 int bldfm_plan_set_surrogate(bldfm_plan* P, const void* net, const bldfm_sur_hull* hull, double tol) {

@@ -395,7 +395,8 @@ __global__ __launch_bounds__(256) void k_sur_gate(PlanDev P, ExecBufs B) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.ninst || B.mode[i] != 0) return;
   const double2 s = B.gate[i];
-  const double err = s.y > 0.0 ? sqrt(s.x / s.y) : 0.0;
+  // no checked lane (or a zero reference) is no evidence: the solver serves it
+  const double err = s.y > 0.0 ? sqrt(s.x / s.y) : INFINITY;
   if (!(err <= P.sur_tol)) B.mode[i] = 1;
 }
 

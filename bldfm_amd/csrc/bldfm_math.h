@@ -160,23 +160,33 @@ enum Closure : int32_t { CL_MOST = 0, CL_CONSTANT = 1, CL_MOSTM = 2, CL_OAAHOC =
 // The profile formulas below round as the reference's numpy/glibc evaluation
 // does: every product and sum on its own, in the reference's order (no
 // contraction), with the correctly rounded or glibc-compatible functions of
-// bldfm_crmath.h in place of the device libm.
+// bldfm_crmath.h in place of the device libm.  EXACT = false swaps in the
+// plain libm functions for callers that round the result to float anyway
+// (the surrogate's input features), where the double-double cost buys nothing.
+template <bool EXACT> BLDFM_HD double pf_log(double x) { return EXACT ? cr_log(x) : log(x); }
+template <bool EXACT> BLDFM_HD double pf_atan(double x) { return EXACT ? cr_atan(x) : atan(x); }
+template <bool EXACT> BLDFM_HD double pf_cpow(double v, double p) {
+  return EXACT ? gl_cpow_real(v, p) : exp(p * log(v));
+}
 
 // MOST momentum stability correction psi(z/L) (src/bldfm/pbl_model.py:207-230):
 // 5x for x > 0, else with xi = Re((1 - 16x)^(1/4)) (complex power)
 // -2 ln((1 + xi)/2) - ln((1 + xi^2)/2) + 2 atan(xi) - pi/2.
+template <bool EXACT = true>
 BLDFM_HD double most_psi(double x) {
   CR_EXACT
   if (x > 0.0) return 5.0 * x;
-  const double xi = gl_cpow_real(1.0 - 16.0 * x, 0.25);
-  return -2.0 * cr_log(0.5 * (1.0 + xi)) - cr_log(0.5 * (1.0 + xi * xi)) + 2.0 * cr_atan(xi) - 0.5 * BLDFM_PI;
+  const double xi = pf_cpow<EXACT>(1.0 - 16.0 * x, 0.25);
+  return -2.0 * pf_log<EXACT>(0.5 * (1.0 + xi)) - pf_log<EXACT>(0.5 * (1.0 + xi * xi)) + 2.0 * pf_atan<EXACT>(xi) -
+         0.5 * BLDFM_PI;
 }
 
 // MOST diffusivity stability function phi(z/L) (src/bldfm/pbl_model.py:233-250).
+template <bool EXACT = true>
 BLDFM_HD double most_phi(double x) {
   CR_EXACT
   if (x > 0.0) return 1.0 + 5.0 * x;
-  return gl_cpow_real(1.0 - 16.0 * x, -0.5);
+  return pf_cpow<EXACT>(1.0 - 16.0 * x, -0.5);
 }
 
 // Met wind (speed, direction blowing FROM, degrees) -> (u, v)
@@ -267,20 +277,21 @@ BLDFM_HD int prof_setup_given(int n, double zm, double um, double vm, double ust
 
 // Height and profile values at grid point i (src/bldfm/pbl_model.py:127-182).
 // out = {z, u, v, Kx, Ky, Kz}
+template <bool EXACT = true>
 BLDFM_HD void prof_point(const ProfSetup& s, int i, double out[6]) {
   CR_EXACT
   const double kap = BLDFM_KARMAN;
   const double zeta = 0.0 + i * s.dzeta;
-  const double z = -s.h * cr_log(-(zeta - s.aa) / s.bb);
+  const double z = -s.h * pf_log<EXACT>(-(zeta - s.aa) / s.bb);
   double u, v, kx, ky, kz;
   if (s.closure == CL_CONSTANT) {
     const double km = kap * s.ustar * s.zm / s.prsc;
     u = s.um; v = s.vm; kx = ky = kz = km;
   } else if (s.closure == CL_MOST || s.closure == CL_MOSTM) {
-    const double absu = s.ustar / kap * (cr_log(z / s.z0) + most_psi(z / s.mol));
+    const double absu = s.ustar / kap * (pf_log<EXACT>(z / s.z0) + most_psi<EXACT>(z / s.mol));
     u = s.um / s.absum * absu;
     v = s.vm / s.absum * absu;
-    const double K = kap * s.ustar * z / most_phi(z / s.mol) / s.prsc;
+    const double K = kap * s.ustar * z / most_phi<EXACT>(z / s.mol) / s.prsc;
     if (s.closure == CL_MOST) {
       kx = ky = kz = K;
     } else {
@@ -290,7 +301,7 @@ BLDFM_HD void prof_point(const ProfSetup& s, int i, double out[6]) {
     }
   } else {  // OAAHOC
     const double cl = 0.845, cm = 0.0856, ch = 0.204;
-    const double absu = s.ustar * s.ustar / cm / cl / sqrt(s.tke) * cr_log(z / s.z0);
+    const double absu = s.ustar * s.ustar / cm / cl / sqrt(s.tke) * pf_log<EXACT>(z / s.z0);
     u = s.um / s.absum * absu;
     v = s.vm / s.absum * absu;
     kx = ky = kz = ch * cl * z * sqrt(s.tke);

@@ -59,3 +59,31 @@ def prologue(meas_height, wind, ustar, z0, mol, closure, domain_height, stretch,
     zetamx = np.squeeze(zetamx).item()
     return (float(np.squeeze(z0)), float(np.squeeze(ustar)), float(np.squeeze(aa)), float(np.squeeze(bb)),
             float(zetamx))
+
+
+def prologue_many(meas_height, um, vm, ustar, z0, mol, closure, domain_height, stretch):
+    """``prologue`` for many MOST-type instances at once (arrays of equal
+    length; ``ustar`` or ``z0`` None for all of them, ``domain_height`` and
+    ``stretch`` arrays or None).  numpy evaluates each element as it
+    evaluates a scalar, so the values are those ``prologue`` gives one
+    instance at a time.  Returns arrays (z0, ustar, aa, bb, zeta_max)."""
+    zm = np.asarray(meas_height, dtype=np.float64)
+    um, vm = np.asarray(um, dtype=np.float64), np.asarray(vm, dtype=np.float64)
+    mol = np.asarray(mol, dtype=np.float64)
+    kap = KARMAN
+    absum = np.sqrt(um**2 + vm**2)
+    if closure not in ("CONSTANT", "MOST", "MOSTM") or (ustar is None) == (z0 is None):
+        raise ValueError("prologue_many serves MOST-type closures with exactly one of ustar/z0")
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        if z0 is None:
+            ustar = np.asarray(ustar, dtype=np.float64)
+            z0 = zm * np.exp(-kap * absum / ustar + psi(zm / mol))
+        else:
+            z0 = np.asarray(z0, dtype=np.float64)
+            ustar = absum * kap / (np.log(zm / z0) + psi(zm / mol))
+        h = 2.0 * zm if stretch is None else np.asarray(stretch, dtype=np.float64)
+        zmx = 2.0 * zm if domain_height is None else np.asarray(domain_height, dtype=np.float64)
+        bb = zm / (np.exp(-z0 / h) - np.exp(-zm / h))
+        aa = bb * np.exp(-z0 / h)
+        zetamx = aa - bb * np.exp(-zmx / h)
+    return z0, ustar, aa, bb, zetamx

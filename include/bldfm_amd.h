@@ -107,6 +107,13 @@ int bldfm_abi_version(void);
 /* Last error message of the calling thread. */
 const char* bldfm_last_error(void);
 
+/* Device memory: bytes the library itself holds on `device` (the FFT plan
+ * and twiddle blocks of bldfm_fft2, an LRU cache capped at 16 MiB; plans are
+ * the caller's, see bldfm_plan_destroy) and hipMemGetInfo's free/total.
+ * No counterpart in the reference; it serves the memory checks that follow
+ * tests/test_memory.py:64-119 of the reference.  Null pointers are skipped. */
+int bldfm_mem_info(int device, size_t* cache_bytes, size_t* free_bytes, size_t* total_bytes);
+
 /*
  * Replaces the scalar prologue of vertical_profiles
  * (src/bldfm/pbl_model.py:58-129): number of grid points and the derived
@@ -257,6 +264,7 @@ int bldfm_math_probe(int32_t op, const double* a, const double* b, int64_t n, do
  * stays its fallback.
  */
 #define BLDFM_SUR_SPOT_STRIDE 100   /* one spot-checked mode in 100 (1 %) */
+#define BLDFM_SUR_MIN_SPOT 16       /* instances with fewer spot-checked modes are always solved */
 
 /* Box of met conditions and the vertical-grid convention a network was
  * trained on.  Instances outside it are always solved. */
@@ -321,7 +329,9 @@ int bldfm_sur_adam(float* params, float* grad, float* m, float* v, void* net, fl
  * `net` (null: none): each execution evaluates the network on every mode of
  * those inside `hull`, solves one mode in BLDFM_SUR_SPOT_STRIDE in FP64,
  * and solves the whole instance instead when the spot-check relative L2
- * error exceeds tol.  Served instances get the flux footprint; their conc
+ * error exceeds tol or no spot-checked mode carried a nonzero reference.
+ * Instances with fewer than BLDFM_SUR_MIN_SPOT spot-checked modes are
+ * outside every hull.  Served instances get the flux footprint; their conc
  * output is 0. */
 int bldfm_plan_set_surrogate(bldfm_plan* plan, const void* net, const bldfm_sur_hull* hull, double tol);
 

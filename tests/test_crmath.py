@@ -311,3 +311,25 @@ def test_profiles_met_sweep(shim):
             assert np.all(np.isclose(a, b, rtol=1e-12, atol=1e-15))
     print(f"\n  {cases} met cases, {values} profile values, {bad} differ ({bad / values:.2%})")
     assert cases >= 200 and bad / values < 0.01
+
+
+def test_vectorised_prologue_equals_scalar_prologue():
+    """The batched drop-ins evaluate the numpy prologue over whole batches
+    (bldfm_amd.prologue.prologue_many); each element equals the one-instance
+    evaluation bit for bit, for both the u*-given and the z0-given forms."""
+    from bldfm_amd.prologue import prologue, prologue_many
+
+    r = np.random.default_rng(12)
+    n = 3000
+    zm, us, z0 = r.uniform(2, 50, n), r.uniform(0.1, 1.0, n), 10 ** r.uniform(-2, -0.5, n)
+    mol = 1.0 / np.where(np.abs(iv := r.uniform(-0.1, 0.05, n)) < 1e-6, 1e-9, iv)
+    sp, wd = r.uniform(1, 10, n), np.deg2rad(r.uniform(0, 360, n))
+    um, vm = -sp * np.sin(wd), -sp * np.cos(wd)
+    dh = zm * r.uniform(2, 8, n)
+    for given in ("ustar", "z0"):
+        many = prologue_many(zm, um, vm, us if given == "ustar" else None, z0 if given == "z0" else None, mol,
+                             "MOST", dh, None)
+        for i in range(0, n, 7):
+            one = prologue(float(zm[i]), (float(um[i]), float(vm[i])), float(us[i]) if given == "ustar" else None,
+                           float(z0[i]) if given == "z0" else None, float(mol[i]), "MOST", float(dh[i]), None, None)
+            assert tuple(float(a[i]) for a in many) == one or any(np.isnan(one)), (given, i)

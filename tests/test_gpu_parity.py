@@ -58,39 +58,43 @@ DOUBLE_CASES = ["source_area", "plume_3d", "single_footprint", "c1_dispersion", 
 
 @pytest.mark.parametrize("name", ["c2_unstable_dispersion", "c2_stable_dispersion"])
 def test_c2_three_dimensional_field(gpu, name):
-    """Production-size 3-D dispersion field (every one of the 65 levels) of
-    the 16-Gaussian source: the GPU's all-level solve against the oracle on
-    five of those levels (each level's output is independent of which other
-    levels are requested), and the top of the field against its bottom.
+    """Production-size 3-D dispersion field of the 16-Gaussian source: every
+    one of the 65 levels of the GPU's all-level solve against the oracle's.
 
-    Bar per level: 1e-10, or 8x the level's own sensitivity to the last bit
-    of the shooting coefficient, whichever is larger.  The sensitivity is the
-    oracle's output change when alpha moves by one ulp on the same inputs:
-    near the top of this 108 m field the growing and decaying solutions of
-    the retained modes (|k| H up to 29) cancel, so the reference's own value
+    Bars: rel-L2 <= 1e-10 on every level up to the measurement height
+    (levels 0..26).  Above it, 1e-10 or 8x the level's own sensitivity to the
+    last bit of the shooting coefficient, whichever is larger: the oracle's
+    output change when alpha moves by one ulp on the same inputs.  Near the
+    top of this 108 m column the growing and decaying solutions of the
+    retained modes (|k| H up to 29) cancel, so the reference's own value
     there depends on alpha's last bit and no implementation -- nor the
-    reference on another CPU -- can agree with it more closely.  Unstable
-    case: 1.3e-6 (conc) and 6e-4 (flx) at level 64, below 1e-10 up to level
-    45.  Stable case: 6e-7 / 2.5e-4 at level 45 and about 3 at level 64,
-    i.e. the reference's top level is roundoff noise there."""
+    reference on another CPU -- can agree with it more closely.  Levels whose
+    one-ulp sensitivity exceeds 1e-3 are printed as "noise": the reference's
+    own output there is roundoff."""
     _, solver = gpu
     print(f"[c2 3-D] {name}", flush=True)
     q0, z, profs, domain, levels, kw = scenarios.inputs(name, opbl.vertical_profiles)
     assert len(z) == 65 and levels == list(range(65))
     _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
     assert gc.shape == (65, 512, 512) and np.isfinite(gc).all() and np.isfinite(gf).all()
-    sub = [0, 13, 26, 45, 64]
-    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, sub, **kw)
-    _, pc, pf = osolver.steady_state_transport_solver(q0, z, profs, domain, sub, _alpha_ulps=1, **kw)
-    for j, lv in enumerate(sub):
-        ec, ef = rel_l2(gc[lv], oc[j]), rel_l2(gf[lv], of[j])
-        bc, bf = max(1e-10, 8 * rel_l2(pc[j], oc[j])), max(1e-10, 8 * rel_l2(pf[j], of[j]))
-        print(f"  level {lv}: rel-L2 conc {ec:.2e} (bar {bc:.1e}) flx {ef:.2e} (bar {bf:.1e})")
-        assert ec <= bc and ef <= bf
-        if lv <= 26:  # up to the measurement height the plain 1e-10 bar holds
-            assert ec <= 1e-10 and ef <= 1e-10
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, **kw)
+    _, pc, pf = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, _alpha_ulps=1, **kw)
+    zm_level = 26
+    worst_below = 0.0
+    for lv in range(65):
+        ec, ef = rel_l2(gc[lv], oc[lv]), rel_l2(gf[lv], of[lv])
+        sc, sf = rel_l2(pc[lv], oc[lv]), rel_l2(pf[lv], of[lv])
+        if lv <= zm_level:
+            bc = bf = 1e-10
+            worst_below = max(worst_below, ec, ef)
+        else:
+            bc, bf = max(1e-10, 8 * sc), max(1e-10, 8 * sf)
+        tag = " noise" if max(sc, sf) > 1e-3 else ""
+        print(f"  level {lv:2d} z={z[lv]:7.2f} m: conc {ec:.1e} (bar {bc:.0e}) flx {ef:.1e} (bar {bf:.0e}){tag}")
+        assert ec <= bc and ef <= bf, lv
+    print(f"  worst rel-L2 up to zm: {worst_below:.2e}")
     # up to the measurement height the plume of surface sources thins out
-    assert gc[26].max() < gc[1].max()
+    assert gc[zm_level].max() < gc[1].max()
 
 
 @pytest.mark.parametrize("mol", [-50.0, 1e9, 200.0])

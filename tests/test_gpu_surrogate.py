@@ -9,6 +9,8 @@ Bars: point solver vs oracle transfer rel-L2 <= 1e-8 (FP64, unit instead of
 fallback footprints bitwise equal to the plain batch.
 """
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -179,6 +181,32 @@ def test_gate_falls_back_to_the_solver(net):
     assert mode == [1, 1, 1]
     assert err[0] > 1e-3 and err[1] > 1e-3 and np.isnan(err[2])
     for i in range(len(specs)):
+        for a, b in zip(fb.output(i), plain.output(i)):
+            assert torch.equal(a, b)
+
+
+def test_small_instances_are_never_served_unchecked(net):
+    """Instances with few modes get fewer than BLDFM_SUR_MIN_SPOT spot-checked
+    modes (with 8x8 modes, possibly none at all): whatever the tolerance and
+    wherever they sit in the batch, the solver serves them, bit for bit; a
+    large instance beside them is still served by the network."""
+    import torch
+
+    from bldfm_amd import surrogate as sur
+    from bldfm_amd.batch import FootprintBatch
+
+    untrained = sur.SurrogateNet(net.static.numpy()[132:140], net.static.numpy()[140:148], seed=1)
+    base = _specs([10.0, 12.0, 14.0, 15.0, 18.0])
+    modes = [(8, 8), (16, 16), (128, 128), (32, 32), (8, 8)]
+    specs = [dataclasses.replace(s, modes=m) for s, m in zip(base, modes)]
+    plain = FootprintBatch(specs)
+    plain.run()
+    fb = FootprintBatch(specs, surrogate=untrained, tol=1e9)
+    fb.run()
+    torch.cuda.synchronize()
+    err, mode = fb.surrogate_stats()
+    assert mode == [1, 1, 0, 1, 1], (err, mode)
+    for i in (0, 1, 3, 4):
         for a, b in zip(fb.output(i), plain.output(i)):
             assert torch.equal(a, b)
 
