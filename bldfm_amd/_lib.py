@@ -123,6 +123,7 @@ SIGNATURES = {
     "bldfm_plan_sizes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "bldfm_plan_offsets": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
     "bldfm_plan_execute": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "bldfm_plan_execute_static": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "bldfm_plan_timing_begin": (ctypes.c_int, [_P, _I32]),
     "bldfm_plan_timing_end": (ctypes.c_int, [_P, ctypes.POINTER(_D), ctypes.POINTER(_I32)]),
     "bldfm_stability": (ctypes.c_int, [_P, _I64, _P, _P, _P]),
@@ -343,6 +344,14 @@ class Plan:
             lib.bldfm_plan_execute(self.handle, ptr(prof), ptr(src), ptr(conc), ptr(flx), ptr(workspace), s),
             "plan_execute",
         )
+
+    def execute_static(self, prof, src, conc, flx, stream=None):
+        """``execute`` with the library's static per-GPU scratch as workspace
+        (no device allocation; plans up to BLDFM_STATIC_SCRATCH_BYTES of
+        workspace; overlapping calls on one GPU race)."""
+        s = stream if stream is not None else stream_handle(self.device)
+        check(load().bldfm_plan_execute_static(self.handle, ptr(prof), ptr(src), ptr(conc), ptr(flx), s),
+              "plan_execute_static")
 
     PHASES = ("level_table", "forward_fft", "surrogate", "shoot", "nyquist_fix", "inverse_y", "inverse_x")
 

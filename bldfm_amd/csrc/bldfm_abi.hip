@@ -649,6 +649,54 @@ int bldfm_plan_offsets(const bldfm_plan* P, int32_t i, int64_t* off_prof, int64_
   return 0;
 }
 
+}  // extern "C"
+
+// Workspace of bldfm_plan_execute_static: one array per GPU the library runs
+// on, held until the process ends.  Host code never uses the array's name as
+// an address: the device address comes from hipGetSymbolAddress, once per
+// device with that device current, and is kept per device.
+static __device__ __align__(256) char scratch[BLDFM_STATIC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device -> address of that device's `scratch`
+
+// Address of the current device's scratch array (device `dev` must be current).
+int scratch_address(int dev, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    it = g_scratch_ptr.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int bldfm_plan_execute_static(bldfm_plan* P, double* prof, const double* src, void* conc, void* flx,
+                              hipStream_t stream) {
+  if (!P) return fail(BLDFM_EINVAL, "null plan");
+  if (!prof || !conc || !flx) return fail(BLDFM_EINVAL, "null buffer");
+  if (P->ws_bytes > (size_t)BLDFM_STATIC_SCRATCH_BYTES)
+    return fail(BLDFM_ELIMIT, "plan workspace of " + std::to_string(P->ws_bytes) +
+                                  " bytes exceeds the static scratch of " +
+                                  std::to_string((size_t)BLDFM_STATIC_SCRATCH_BYTES) +
+                                  " bytes; use bldfm_plan_execute with a workspace");
+  void* ws = nullptr;
+  {
+    DeviceGuard guard(P->device);
+    if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
+    const int rc = scratch_address(P->device, &ws);
+    if (rc) return rc;
+  }
+  return bldfm_plan_execute(P, prof, src, conc, flx, ws, stream);
+}
+
 int bldfm_plan_execute(bldfm_plan* P, double* prof, const double* src, void* conc, void* flx,
                        void* workspace, hipStream_t stream) {
   if (!P) return fail(BLDFM_EINVAL, "null plan");

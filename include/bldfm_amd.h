@@ -222,6 +222,23 @@ int bldfm_plan_execute(bldfm_plan* plan, double* prof, const double* src, void* 
                        void* workspace, hipStream_t stream);
 
 /*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): bldfm_plan_execute with its workspace taken from a
+ * library-owned static device array of BLDFM_STATIC_SCRATCH_BYTES on each GPU
+ * the library runs on (held until the process ends).  Calls no hipMalloc,
+ * hipFree or allocating routine.  Plans whose workspace (bldfm_plan_sizes)
+ * exceeds the array are refused with BLDFM_ELIMIT.  Every call on one GPU
+ * shares that GPU's array and it keeps the last call's contents (what must
+ * start at zero is cleared by a kernel of each execution): executions that
+ * overlap on one GPU -- other streams, other host threads -- race and must be
+ * serialised by the caller.
+ */
+#define BLDFM_STATIC_SCRATCH_BYTES (64u << 20)
+int bldfm_plan_execute_static(bldfm_plan* plan, double* prof, const double* src, void* conc, void* flx,
+                              hipStream_t stream);
+
+/*
  * Phase timing (no counterpart in the reference; for benchmarks): after
  * bldfm_plan_timing_begin, the next max_execs executions record HIP events
  * on their stream at the phase boundaries; bldfm_plan_timing_end waits for
