@@ -50,6 +50,7 @@ int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 // ------------------------------------------------------------- FFT planning
 // Radix order: 8s, then one 4 or 2, then 3s, 5s, then remaining primes.
 bool factorize(int n, FftPlan* P) {
+  memset(P, 0, sizeof(*P));
   P->n = n;
   P->nstage = 0;
   int m = n;
@@ -122,13 +123,22 @@ int max_fft_len(int device) {
   return n;
 }
 
-int check_fft_len(int n, int device, const char* what) {
-  if (n > max_fft_len(device))
+// Lengths above one workgroup's LDS limit run as four-step transforms of two
+// factors that fit; a length without such a factor pair (a prime above the
+// limit, say) is refused.  Returns the split factor n1 (0: no split) or -1.
+int check_fft_len(int n, int device, const char* what, int* split = nullptr) {
+  const int limit = max_fft_len(device);
+  const int s = fft_split(n, limit);
+  if (s < 0)
     return fail(BLDFM_ELIMIT, std::string(what) + " = " + std::to_string(n) +
-                                  " exceeds the single-workgroup FFT limit (" +
-                                  std::to_string(max_fft_len(device)) + " points)");
+                                  " exceeds the single-workgroup FFT limit (" + std::to_string(limit) +
+                                  " points) and has no split into two factors within it");
+  if (split) *split = s;
   return 0;
 }
+
+// The length whose LDS transform a plan of length n runs: n, or its larger factor.
+int lds_len(int n, int split) { return split > 0 ? std::max(split, n / split) : n; }
 
 // Plans of the plain 2-D FFT, per (device, n): one device block holding the
 // FftPlan followed by its n twiddles.  Least recently used blocks are freed
@@ -181,7 +191,16 @@ int cached_plan(int device, int n, const FftPlan** plan, const double2** tw) {
   auto it = g_tw_cache.find(key);
   if (it == g_tw_cache.end()) {
     FftPlan F;
-    if (!factorize(n, &F)) return fail(BLDFM_ELIMIT, "too many FFT stages");
+    const int split = fft_split(n, max_fft_len(device));
+    if (split < 0) return fail(BLDFM_ELIMIT, "FFT length has no split into two single-workgroup lengths");
+    if (split > 0) {  // four-step: the factors are planned on their own
+      memset(&F, 0, sizeof(F));
+      F.n = n;
+      F.n1 = split;
+      F.n2 = n / split;
+    } else if (!factorize(n, &F)) {
+      return fail(BLDFM_ELIMIT, "too many FFT stages");
+    }
     F.tw_off = 0;
     std::vector<char> h(kPlanHdr + sizeof(double2) * n, 0);
     memcpy(h.data(), &F, sizeof(F));
@@ -361,8 +380,9 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     plan_n.push_back(n);
     return (int)plan_n.size() - 1;
   };
-  // tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows, sur, spot, transpose
-  constexpr int kKinds = 10;
+  // tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows, sur, spot, transpose,
+  // then the four-step steps A, B of fwd_rows, fwd_cols, inv_cols, inv_rows
+  constexpr int kKinds = 18;
   std::vector<int64_t> tiles[kKinds];
   for (auto& v : tiles) v.assign(ninst + 1, 0);
   std::vector<SurMet> surmet(ninst);
@@ -392,10 +412,13 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     if (D.nlx > D.nxe || D.nly > D.nye) return bad(BLDFM_EINVAL, "modes exceed the padded grid");
     if ((D.nxe - D.nlx) % 2 || (D.nye - D.nly) % 2)
       return bad(BLDFM_EINVAL, "padded grid and mode counts must have equal parity.");
-    int rc = check_fft_len(D.nxe, device, "padded x length");
-    if (!rc) rc = check_fft_len(D.nye, device, "padded y length");
+    int split_x = 0, split_y = 0;
+    int rc = check_fft_len(D.nxe, device, "padded x length", &split_x);
+    if (!rc) rc = check_fft_len(D.nye, device, "padded y length", &split_y);
     if (rc) { delete P; return rc; }
-    max_n = std::max(max_n, std::max(D.nxe, D.nye));
+    D.long_x = split_x > 0;
+    D.long_y = split_y > 0;
+    max_n = std::max(max_n, std::max(lds_len(D.nxe, split_x), lds_len(D.nye, split_y)));
     D.nxb = band_len(D.nlx, D.nxe);
     D.nyb = band_len(D.nly, D.nye);
     if (I.prof_from_met) {
@@ -418,6 +441,8 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     if (!D.footprint) any_disp = 1;
     D.plan_x = plan_id(D.nxe);
     D.plan_y = plan_id(D.nye);
+    if (split_x) { plan_id(split_x); plan_id(D.nxe / split_x); }
+    if (split_y) { plan_id(split_y); plan_id(D.nye / split_y); }
     D.lanes = shoot_lanes(D.nlx, D.nly);
     D.dx = I.dx; D.dy = I.dy;
     if (!(I.dx > 0.0) || !(I.dy > 0.0)) return bad(BLDFM_EINVAL, "grid increments must be positive");
@@ -478,22 +503,44 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     D.off_v = off_ws; off_ws += (int64_t)D.nslot * D.nxb * D.ny;
     if (xlayout == 2) { D.off_vt = off_ws; off_ws += (int64_t)D.nslot * D.nxb * D.ny; }
     D.off_out = off_out; off_out += (int64_t)D.nslot * D.ny * D.nx;
+    // four-step scratch: lines x n of the largest split pass
+    int64_t long_elems = 0;
+    if (D.long_x) {
+      long_elems = std::max(long_elems, (int64_t)D.nslot * D.ny * D.nxe);
+      if (!D.footprint) long_elems = std::max(long_elems, (int64_t)D.ny * D.nxe);
+    }
+    if (D.long_y) {
+      long_elems = std::max(long_elems, (int64_t)D.nslot * D.nxb * D.nye);
+      if (!D.footprint) long_elems = std::max(long_elems, (int64_t)D.nlx * D.nye);
+    }
+    D.off_long = long_elems ? off_ws : -1;
+    off_ws += long_elems;
 
     // tiles: tab, fwd_rows, fwd_cols, shoot, nyq, inv_cols, inv_rows
     const int fixc = nyq_fix_count(D.nlx, D.nly, D.nxb, D.nyb);
     const int64_t nspot = (D.lanes + BLDFM_SUR_SPOT_STRIDE - 1) / BLDFM_SUR_SPOT_STRIDE;
+    // lines of each pass; a split axis moves them to the four-step tiles
+    const int64_t lines_fr = D.footprint ? 0 : D.ny, lines_fc = D.footprint ? 0 : D.nlx;
+    const int64_t lines_ic = (int64_t)D.nslot * D.nxb, lines_ir = (int64_t)D.nslot * D.ny;
+    const int64_t x1 = split_x, x2 = split_x ? D.nxe / split_x : 0;
+    const int64_t y1 = split_y, y2 = split_y ? D.nye / split_y : 0;
     const int64_t nt[kKinds] = {
         (D.nz + 255) / 256,
-        D.footprint ? 0 : D.ny,
-        D.footprint ? 0 : D.nlx,
+        D.long_x ? 0 : lines_fr,
+        D.long_y ? 0 : lines_fc,
         (D.lanes + shoot_block - 1) / shoot_block,
         fixc > 0 ? (int64_t)D.nslot * ((fixc + 255) / 256) : 0,
-        (int64_t)D.nslot * D.nxb,
-        (int64_t)D.nslot * D.ny,
+        D.long_y ? 0 : lines_ic,
+        D.long_x ? 0 : lines_ir,
         want_sur ? (D.lanes + SUR_PTS - 1) / SUR_PTS : 0,
         want_sur ? (nspot + 255) / 256 : 0,
         xlayout == 2 ? (int64_t)D.nslot * ((D.nxb + 31) / 32) * ((D.ny + 31) / 32) : 0,
+        // four-step steps A (lines x n2) and B (lines x n1) per pass
+        lines_fr * x2, lines_fr * x1, lines_fc * y2, lines_fc * y1,
+        lines_ic * y2, lines_ic * y1, lines_ir * x2, lines_ir * x1,
     };
+    for (int k = 10; k < kKinds; ++k)
+      if (nt[k] >= ((int64_t)1 << 31)) return bad(BLDFM_ELIMIT, "too many four-step transforms in one instance");
     for (int k = 0; k < kKinds; ++k) tiles[k][i + 1] = tiles[k][i] + nt[k];
   }
   if (lv - levels > (int64_t)1 << 31) return bad(BLDFM_EINVAL, "level table too large");
@@ -520,10 +567,27 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
   // FFT plans and twiddles
   std::vector<FftPlan> plans(plan_n.size());
   int64_t tw_total = 0;
+  const int fft_limit = max_fft_len(device);
+  auto id_of = [&](int n) {
+    for (size_t i = 0; i < plan_n.size(); ++i)
+      if (plan_n[i] == n) return (int)i;
+    return -1;
+  };
   for (size_t k = 0; k < plan_n.size(); ++k) {
-    if (!factorize(plan_n[k], &plans[k])) return bad(BLDFM_ELIMIT, "too many FFT stages");
+    const int n = plan_n[k];
+    const int split = fft_split(n, fft_limit);
+    if (split > 0) {
+      memset(&plans[k], 0, sizeof(FftPlan));
+      plans[k].n = n;
+      plans[k].n1 = split;
+      plans[k].n2 = n / split;
+      plans[k].plan1 = id_of(split);
+      plans[k].plan2 = id_of(n / split);
+    } else if (!factorize(n, &plans[k])) {
+      return bad(BLDFM_ELIMIT, "too many FFT stages");
+    }
     plans[k].tw_off = tw_total;
-    tw_total += plan_n[k];
+    tw_total += n;
   }
   std::vector<double2> tw(tw_total);
   for (size_t k = 0; k < plan_n.size(); ++k) twiddles(plan_n[k], tw.data() + plans[k].tw_off);
@@ -581,6 +645,10 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
   V.t_sur = (const TileRec*)(base + o_tiles[7]);
   V.t_spot = (const TileRec*)(base + o_tiles[8]);
   V.t_tr = (const TileRec*)(base + o_tiles[9]);
+  for (int k = 0; k < 8; ++k) {
+    V.t_long[k] = (const TileRec*)(base + o_tiles[10 + k]);
+    V.tot_long[k] = tiles[10 + k][ninst];
+  }
   V.tot_tr = tiles[9][ninst];
   V.shoot_block = shoot_block;
   V.xlayout = xlayout;
@@ -788,7 +856,11 @@ int bldfm_math_probe(int32_t op, const double* a, const double* b, int64_t n, do
 }
 
 size_t bldfm_fft2_workspace_bytes(int32_t n0, int32_t n1, int32_t batch) {
-  return (size_t)n0 * n1 * batch * sizeof(double2);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int limit = max_fft_len(dev);
+  const bool split = fft_split(n0, limit) != 0 || fft_split(n1, limit) != 0;
+  return (size_t)n0 * n1 * batch * sizeof(double2) * (split ? 2 : 1);
 }
 
 int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* in, double* out,
@@ -798,18 +870,38 @@ int bldfm_fft2(int device, int32_t n0, int32_t n1, int32_t batch, const double* 
   if (sign != 1 && sign != -1) return fail(BLDFM_EINVAL, "sign must be +1 or -1");
   DeviceGuard guard(device);
   if (!guard.ok) return fail(BLDFM_EHIP, "hipSetDevice failed");
-  int rc = check_fft_len(n0, device, "fft2 axis 0 length");
-  if (!rc) rc = check_fft_len(n1, device, "fft2 axis 1 length");
+  int s0 = 0, s1 = 0;
+  int rc = check_fft_len(n0, device, "fft2 axis 0 length", &s0);
+  if (!rc) rc = check_fft_len(n1, device, "fft2 axis 1 length", &s1);
   if (rc) return rc;
-  const FftPlan *F0 = nullptr, *F1 = nullptr;
-  const double2 *tw0 = nullptr, *tw1 = nullptr;
-  rc = cached_plan(device, n0, &F0, &tw0);
-  if (!rc) rc = cached_plan(device, n1, &F1, &tw1);
-  if (rc) return rc;
-  const int nmax = std::max(n0, n1);
+  FftSplit A[2];
+  const int lens[2] = {n0, n1}, splits[2] = {s0, s1};
+  for (int a = 0; a < 2; ++a) {
+    memset(&A[a], 0, sizeof(FftSplit));
+    rc = cached_plan(device, lens[a], &A[a].F, &A[a].tw);
+    if (!rc && splits[a]) {
+      A[a].n1 = splits[a];
+      A[a].n2 = lens[a] / splits[a];
+      rc = cached_plan(device, A[a].n1, &A[a].F1, &A[a].tw1);
+      if (!rc) rc = cached_plan(device, A[a].n2, &A[a].F2, &A[a].tw2);
+    }
+    if (rc) return rc;
+  }
+  {  // a later cached_plan may have evicted an earlier block of this call: look them up again
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    for (int a = 0; a < 2; ++a) {
+      const int ns[3] = {lens[a], A[a].n1, A[a].n2};
+      for (int q = 0; q < (splits[a] ? 3 : 1); ++q)
+        if (g_tw_cache.find(std::make_pair(device, ns[q])) == g_tw_cache.end())
+          return fail(BLDFM_ELIMIT, "fft2 plans exceed the twiddle cache");
+    }
+  }
+  const int nmax = std::max(lds_len(n0, s0), lds_len(n1, s1));
   set_lds_attr(fft_lds_bytes(nmax));
-  rc = bldfm_launch_fft2(F0, n0, tw0, F1, n1, tw1, batch, (const double2*)in, (double2*)workspace,
-                         (double2*)out, sign, scale, fft_threads_for(nmax), fft_lds_bytes(nmax), kGridCap, stream);
+  double2* mid = (double2*)workspace;
+  double2* scr = (s0 || s1) ? mid + (int64_t)n0 * n1 * batch : nullptr;
+  rc = bldfm_launch_fft2(A[0], n0, A[1], n1, batch, (const double2*)in, mid, scr, (double2*)out, sign, scale,
+                         fft_threads_for(nmax), fft_lds_bytes(nmax), kGridCap, stream);
   if (rc) return fail(BLDFM_EHIP, std::string("fft2 launch failed: ") +
                                       hipGetErrorString((hipError_t)(-(rc + 1000))));
   return 0;

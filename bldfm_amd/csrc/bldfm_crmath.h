@@ -119,6 +119,45 @@ BLDFM_HD double dd_round(ddouble a) { return a.hi + a.lo; }
 #define CR_PIO4_LO 0x1.1a62633145c07p-55
 #define CR_TWO_OVER_PI 0x1.45f306dc9c883p-1
 
+// 1/n as double-doubles (hi, lo), n = 1..32: the series coefficients, so that
+// no series step needs a double-double division.  In the unrolled loops below
+// every index is a compile-time constant and the values become immediates.
+static constexpr double CR_INV[64] = {
+    0x1.0000000000000p+0, 0x0.0p+0,
+    0x1.0000000000000p-1, 0x0.0p+0,
+    0x1.5555555555555p-2, 0x1.5555555555555p-56,
+    0x1.0000000000000p-2, 0x0.0p+0,
+    0x1.999999999999ap-3, -0x1.999999999999ap-57,
+    0x1.5555555555555p-3, 0x1.5555555555555p-57,
+    0x1.2492492492492p-3, 0x1.2492492492492p-57,
+    0x1.0000000000000p-3, 0x0.0p+0,
+    0x1.c71c71c71c71cp-4, 0x1.c71c71c71c71cp-58,
+    0x1.999999999999ap-4, -0x1.999999999999ap-58,
+    0x1.745d1745d1746p-4, -0x1.745d1745d1746p-59,
+    0x1.5555555555555p-4, 0x1.5555555555555p-58,
+    0x1.3b13b13b13b14p-4, -0x1.3b13b13b13b14p-58,
+    0x1.2492492492492p-4, 0x1.2492492492492p-58,
+    0x1.1111111111111p-4, 0x1.1111111111111p-60,
+    0x1.0000000000000p-4, 0x0.0p+0,
+    0x1.e1e1e1e1e1e1ep-5, 0x1.e1e1e1e1e1e1ep-61,
+    0x1.c71c71c71c71cp-5, 0x1.c71c71c71c71cp-59,
+    0x1.af286bca1af28p-5, 0x1.af286bca1af28p-59,
+    0x1.999999999999ap-5, -0x1.999999999999ap-59,
+    0x1.8618618618618p-5, 0x1.8618618618618p-59,
+    0x1.745d1745d1746p-5, -0x1.745d1745d1746p-60,
+    0x1.642c8590b2164p-5, 0x1.642c8590b2164p-60,
+    0x1.5555555555555p-5, 0x1.5555555555555p-59,
+    0x1.47ae147ae147bp-5, -0x1.eb851eb851eb8p-61,
+    0x1.3b13b13b13b14p-5, -0x1.3b13b13b13b14p-59,
+    0x1.2f684bda12f68p-5, 0x1.2f684bda12f68p-59,
+    0x1.2492492492492p-5, 0x1.2492492492492p-59,
+    0x1.1a7b9611a7b96p-5, 0x1.1a7b9611a7b96p-61,
+    0x1.1111111111111p-5, 0x1.1111111111111p-61,
+    0x1.0842108421084p-5, 0x1.0842108421084p-60,
+    0x1.0000000000000p-5, 0x0.0p+0,
+};
+BLDFM_HD ddouble dd_inv(int n) { return dd_make(CR_INV[2 * (n - 1)], CR_INV[2 * (n - 1) + 1]); }
+
 // e^x as a double-double (|x| <= 700).
 BLDFM_HD ddouble dd_exp(double x) {
   CR_EXACT
@@ -134,10 +173,8 @@ BLDFM_HD ddouble dd_exp(double x) {
   const ddouble s = dd_make(r.hi * 0x1p-8, r.lo * 0x1p-8);
   ddouble e = dd_make(0.0, 0.0);
   // Horner: e = s(1 + s/2(1 + s/3(...)))
-  for (int n = 11; n >= 1; --n) {
-    e = dd_mul(e, s);
-    e = dd_div(dd_add_d(e, 1.0), dd_make((double)n, 0.0));
-  }
+#pragma unroll
+  for (int n = 11; n >= 1; --n) e = dd_mul(dd_add_d(dd_mul(e, s), 1.0), dd_inv(n));
   e = dd_mul(e, s);  // expm1(s) = s (1 + s/2 (...)) with the 1/n folded in
   // undo the scaling: (1 + e)^2 = 1 + (2e + e^2), eight times
   for (int i = 0; i < 8; ++i) e = dd_add(dd_make(2.0 * e.hi, 2.0 * e.lo), dd_mul(e, e));
@@ -147,13 +184,15 @@ BLDFM_HD ddouble dd_exp(double x) {
 }
 
 // 2 atanh(u) = u (2 + 2u^2/3 + 2u^4/5 + ...) for small |u| (double-double u)
-BLDFM_HD ddouble dd_atanh2_series(ddouble u, int terms) {
+// (|u| < 0.032: 14 terms reach 2^-106 relative)
+BLDFM_HD ddouble dd_atanh2_series(ddouble u) {
   CR_EXACT
   const ddouble u2 = dd_mul(u, u);
   ddouble s = dd_make(0.0, 0.0);
-  for (int n = terms - 1; n >= 0; --n) {
-    s = dd_mul(s, u2);
-    s = dd_add(s, dd_div(dd_make(2.0, 0.0), dd_make((double)(2 * n + 1), 0.0)));
+#pragma unroll
+  for (int n = 13; n >= 0; --n) {
+    const ddouble c = dd_inv(2 * n + 1);
+    s = dd_add(dd_mul(s, u2), dd_make(2.0 * c.hi, 2.0 * c.lo));
   }
   return dd_mul(s, u);
 }
@@ -182,7 +221,7 @@ BLDFM_HD double cr_log(double x) {
     const double d = x - 1.0;
     if (d == 0.0) return 0.0;
     const ddouble u = dd_div(dd_make(d, 0.0), dd_two_sum(2.0, d));
-    return dd_round(dd_atanh2_series(u, 14));
+    return dd_round(dd_atanh2_series(u));
   }
   return dd_round(dd_log_newton(dd_make(x, 0.0), log(x)));
 }
@@ -193,7 +232,7 @@ BLDFM_HD double cr_log1p(double d) {
   if (d == 0.0) return d;
   if (fabs(d) < 0.0625) {
     const ddouble u = dd_div(dd_make(d, 0.0), dd_two_sum(2.0, d));
-    return dd_round(dd_atanh2_series(u, 14));
+    return dd_round(dd_atanh2_series(u));
   }
   return dd_round(dd_log_newton(dd_two_sum(1.0, d), log1p(d)));
 }
@@ -224,10 +263,10 @@ BLDFM_HD double cr_atan(double x) {
   // atan u = u - u^3/3 + u^5/5 - ...  (|u| < 0.052)
   const ddouble u2 = dd_mul(t, t);
   ddouble s = dd_make(0.0, 0.0);
+#pragma unroll
   for (int n = 13; n >= 0; --n) {
-    s = dd_mul(s, u2);
-    const ddouble c = dd_div(dd_make(n % 2 ? -1.0 : 1.0, 0.0), dd_make((double)(2 * n + 1), 0.0));
-    s = dd_add(s, c);
+    const ddouble c = dd_inv(2 * n + 1);
+    s = dd_add(dd_mul(s, u2), n % 2 ? dd_neg(c) : c);
   }
   s = dd_mul(s, t);
   const ddouble r = dd_add(base, dd_make(8.0 * sgn * s.hi, 8.0 * sgn * s.lo));
@@ -251,9 +290,10 @@ BLDFM_HD void cr_sincos(double x, double* sn, double* cs) {
   // cos r = C, C = 1 - r^2/(1*2) (1 - r^2/(3*4) (1 - ...))  (|r| <= pi/4, 15 terms)
   const ddouble r2 = dd_mul(r, r);
   ddouble s = dd_make(1.0, 0.0), c = dd_make(1.0, 0.0);
+#pragma unroll
   for (int n = 15; n >= 1; --n) {
-    s = dd_add_d(dd_neg(dd_div(dd_mul(r2, s), dd_make((double)(2 * n) * (double)(2 * n + 1), 0.0))), 1.0);
-    c = dd_add_d(dd_neg(dd_div(dd_mul(r2, c), dd_make((double)(2 * n - 1) * (double)(2 * n), 0.0))), 1.0);
+    s = dd_add_d(dd_neg(dd_mul(dd_mul(dd_mul(r2, s), dd_inv(2 * n)), dd_inv(2 * n + 1))), 1.0);
+    c = dd_add_d(dd_neg(dd_mul(dd_mul(dd_mul(r2, c), dd_inv(2 * n - 1)), dd_inv(2 * n))), 1.0);
   }
   s = dd_mul(s, r);
   const int q = ((int)fmod(k, 4.0) + 4) % 4;
@@ -282,10 +322,8 @@ BLDFM_HD void gl_exp_table(int k, double* H, double* T) {
   const ddouble a = dd_mul_d(dd_make(CR_LN2_HI, CR_LN2_MID), (double)k * 0x1p-7);
   const ddouble s = dd_make(a.hi * 0x1p-8, a.lo * 0x1p-8);
   ddouble e = dd_make(0.0, 0.0);
-  for (int n = 11; n >= 1; --n) {
-    e = dd_mul(e, s);
-    e = dd_div(dd_add_d(e, 1.0), dd_make((double)n, 0.0));
-  }
+#pragma unroll
+  for (int n = 11; n >= 1; --n) e = dd_mul(dd_add_d(dd_mul(e, s), 1.0), dd_inv(n));
   e = dd_mul(e, s);
   for (int i = 0; i < 8; ++i) e = dd_add(dd_make(2.0 * e.hi, 2.0 * e.lo), dd_mul(e, e));
   const ddouble v = dd_add_d(e, 1.0);

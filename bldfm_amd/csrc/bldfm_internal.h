@@ -22,6 +22,10 @@ struct PlanDev {
   const TileRec* t_nyq;
   const TileRec* t_inv_cols;
   const TileRec* t_inv_rows;
+  // four-step lines of split lengths: [2*pass + step] for the PassId order
+  // (forward rows, forward columns, inverse y-pass, inverse x-pass), step A/B
+  const TileRec* t_long[8];
+  int64_t tot_long[8];
   int32_t ninst;
   int32_t any_dispersion;
   int32_t variant_used[4][2]; // shooting kernels this plan launches [variant][exact]
@@ -86,11 +90,23 @@ struct Km01Dev {
   int32_t nx, ny, has_wd, valid;
 };
 int bldfm_launch_km01(const Km01Dev* P, int n, int64_t max_cells, double* out, hipStream_t stream);
-// F0/F1: device-resident plans of the two axis lengths n0 (columns) and n1 (rows)
-int bldfm_launch_fft2(const FftPlan* F0, int32_t n0, const double2* tw0, const FftPlan* F1, int32_t n1,
-                      const double2* tw1, int32_t batch, const double2* in, double2* mid, double2* out,
-                      int32_t sign, double scale, int threads, int lds_bytes, int grid_cap,
-                      hipStream_t stream);
+// Device-resident plan of one axis of the plain 2-D FFT: F (n entries of
+// twiddles at tw) and, for a split length (F1 non-null), the plans of its
+// factors n1, n2 with their twiddles.
+struct FftSplit {
+  const FftPlan* F;
+  const double2* tw;
+  const FftPlan* F1;
+  const double2* tw1;
+  const FftPlan* F2;
+  const double2* tw2;
+  int32_t n1, n2;
+};
+// A0/A1: the axis lengths n0 (columns) and n1 (rows); scr: batch*n0*n1
+// values of four-step scratch when an axis is split (else unused)
+int bldfm_launch_fft2(const FftSplit& A0, int32_t n0, const FftSplit& A1, int32_t n1, int32_t batch,
+                      const double2* in, double2* mid, double2* scr, double2* out, int32_t sign, double scale,
+                      int threads, int lds_bytes, int grid_cap, hipStream_t stream);
 void bldfm_set_fft_lds_attr(int bytes);
 int bldfm_launch_stability(const double* x, int64_t n, double* psi, double* phi, hipStream_t stream);
 int bldfm_launch_wind(const double* spd, const double* dir, int64_t n, double* u, double* v, hipStream_t stream);

@@ -83,26 +83,43 @@ __device__ __forceinline__ void fill_level_row(const double* cur, const double* 
 
 __global__ __launch_bounds__(256) void k_level_table(PlanDev P, ExecBufs B) {
 #pragma clang fp contract(off)
+  // levels evaluated from met in this tile: 256 of them and the one above
+  __shared__ double lev[6][257];
   for (int64_t t = blockIdx.x; t < P.tot_tab; t += gridDim.x) {
     const TileRec tr = tile_rec(P.t_tab, t);
     const int inst = tr.inst;
     const cdesc_t& D = *(const cdesc_t*)(P.desc + inst);
     const int nz = D.nz;
-    const int i = tr.local * (int)blockDim.x + (int)threadIdx.x;
-    if (i >= nz) continue;
+    const int tid = (int)threadIdx.x, T = (int)blockDim.x;
+    const int i = tr.local * T + tid;
     double* prof = B.prof + D.off_prof;
     double cur[6], nxt[6];
-    if (P.met_flag[inst]) {
+    if (P.met_flag[inst]) {  // uniform over the workgroup (one instance per tile)
+      // every level once (the correctly rounded profile functions are the
+      // costly part): each thread its level, the last thread also the next
       const ProfSetup& S = P.met[inst];
-      prof_point(S, i, cur);
-      for (int c = 0; c < 6; ++c) prof[(int64_t)c * nz + i] = cur[c];
-      if (i + 1 < nz) prof_point(S, i + 1, nxt);
+      if (t != blockIdx.x) __syncthreads();  // the previous tile's reads of lev are done
+      if (i < nz) {
+        prof_point(S, i, cur);
+        for (int c = 0; c < 6; ++c) {
+          prof[(int64_t)c * nz + i] = cur[c];
+          lev[c][tid] = cur[c];
+        }
+      }
+      if (tid == T - 1 && i + 1 < nz) {
+        prof_point(S, i + 1, nxt);
+        for (int c = 0; c < 6; ++c) lev[c][T] = nxt[c];
+      }
+      __syncthreads();
+      if (i + 1 < nz)
+        for (int c = 0; c < 6; ++c) nxt[c] = lev[c][tid + 1];
     } else {
-      for (int c = 0; c < 6; ++c) cur[c] = prof[(int64_t)c * nz + i];
+      if (i < nz)
+        for (int c = 0; c < 6; ++c) cur[c] = prof[(int64_t)c * nz + i];
       if (i + 1 < nz)
         for (int c = 0; c < 6; ++c) nxt[c] = prof[(int64_t)c * nz + i + 1];
     }
-    fill_level_row(cur, nxt, i == nz - 1, B.tab + D.off_tab + (int64_t)i * BLDFM_LEVEL_STRIDE);
+    if (i < nz) fill_level_row(cur, nxt, i == nz - 1, B.tab + D.off_tab + (int64_t)i * BLDFM_LEVEL_STRIDE);
   }
 }
 
@@ -514,87 +531,165 @@ __device__ __forceinline__ void fft_dir(bool inv, const cplan_t& F, const double
   else bldfm_fft::lds_fft<MAXE, false>(buf, (const FftPlan*)&F, tw + F.tw_off, tid, T, in, out, reuse);
 }
 
+// The four passes, each as a factory of (plan, source, sink, direction) for
+// one line of one instance: the single-workgroup kernels below and the
+// four-step kernels (k_long) of split lengths run the same definitions.
+enum PassId { PASS_FWD_ROWS = 0, PASS_FWD_COLS = 1, PASS_INV_COLS = 2, PASS_INV_ROWS = 3 };
+
 // Forward, rows: one surface-flux row y placed at [px, px+nx) of nxe points,
 // e^{-i} DFT, scale 1/nxe (numpy norm="forward"), keep the nlx retained
 // columns in truncated order.
-template <int MAXE, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_fwd_rows(PlanDev P, ExecBufs B) {
-  const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t b = blockIdx.x; b < P.tot_fwd_rows; b += gridDim.x) {
-    const int64_t t = gridDim.x >= P.tot_fwd_rows ? xcd_swizzle(b, P.tot_fwd_rows) : b;
-    const TileRec tr = tile_rec(P.t_fwd_rows, t);
-    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
-    const int y = tr.local;
-    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
-    const PadIn<double> in{B.src + D.off_src + (int64_t)y * D.nx, 1, D.px, D.nx};
-    const BandOut out{B.ws + D.off_aw + (int64_t)y * D.nlx, 1.0 / (double)F.n, Band(D.nlx, F.n)};
-    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out, b != blockIdx.x);
-  }
+template <class Fn>
+__device__ __forceinline__ void pass_fwd_rows(const PlanDev& P, const ExecBufs& B, const cdesc_t& D, int64_t line,
+                                              Fn&& fn) {
+  const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
+  const int y = (int)line;
+  fn(F, PadIn<double>{B.src + D.off_src + (int64_t)y * D.nx, 1, D.px, D.nx},
+     BandOut{B.ws + D.off_aw + (int64_t)y * D.nlx, 1.0 / (double)F.n, Band(D.nlx, F.n)}, false);
 }
 
 // Forward, columns: retained column jx over y placed at [py, py+ny) of nye
 // points, e^{-i} DFT, scale 1/nye, keep the nly retained rows -> q^0 (x-major).
-template <int MAXE, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_fwd_cols(PlanDev P, ExecBufs B) {
-  const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t b = blockIdx.x; b < P.tot_fwd_cols; b += gridDim.x) {
-    const int64_t t = gridDim.x >= P.tot_fwd_cols ? xcd_swizzle(b, P.tot_fwd_cols) : b;
-    const TileRec tr = tile_rec(P.t_fwd_cols, t);
-    const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
-    const int jx = tr.local;
-    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
-    const PadIn<double2> in{B.ws + D.off_aw + jx, D.nlx, D.py, D.ny};
-    const BandOut out{B.ws + D.off_q0 + (int64_t)jx * D.nly, 1.0 / (double)F.n, Band(D.nly, F.n)};
-    fft_dir<MAXE>(false, F, P.tw, tid, T, in, out, b != blockIdx.x);
+template <class Fn>
+__device__ __forceinline__ void pass_fwd_cols(const PlanDev& P, const ExecBufs& B, const cdesc_t& D, int64_t line,
+                                              Fn&& fn) {
+  const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
+  const int jx = (int)line;
+  fn(F, PadIn<double2>{B.ws + D.off_aw + jx, D.nlx, D.py, D.ny},
+     BandOut{B.ws + D.off_q0 + (int64_t)jx * D.nly, 1.0 / (double)F.n, Band(D.nly, F.n)}, false);
+}
+
+// Inverse, y-pass: band column cx of slot s (line s*nxb + cx; compact rows
+// scattered to their padded indices), unnormalised DFT (e^{-i} footprint,
+// e^{+i} dispersion), keep the cropped rows [py, py+ny).
+template <class Fn>
+__device__ __forceinline__ void pass_inv_cols(const PlanDev& P, const ExecBufs& B, const cdesc_t& D, int64_t line,
+                                              Fn&& fn) {
+  const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
+  const BandIn in{B.ws + D.off_z + line * D.nyb, 1, Band(D.nyb, F.n)};
+  // layout 1: column cx of slot s lands at v[s][y][cx]
+  const int64_t s = line / D.nxb, cx = line % D.nxb;
+  const CropOut out = P.xlayout == 1 ? CropOut{B.ws + D.off_v + s * D.ny * D.nxb + cx, D.nxb, D.py, D.ny}
+                                     : CropOut{B.ws + D.off_v + line * D.ny, 1, D.py, D.ny};
+  fn(F, in, out, !D.footprint);
+}
+
+// Inverse, x-pass: cropped row y of slot s (line s*ny + y) over the band
+// columns, DFT, keep [px, px+nx); conc = Re, flx = Im.  Instances served by
+// the surrogate carry no concentration spectrum (conc 0).
+template <class Fn>
+__device__ __forceinline__ void pass_inv_rows(const PlanDev& P, const ExecBufs& B, const cdesc_t& D, int inst,
+                                              int64_t line, Fn&& fn) {
+  const int s = (int)(line / D.ny), y = (int)(line % D.ny);
+  const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
+  const BandIn in = P.xlayout == 0
+                        ? BandIn{B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, Band(D.nxb, F.n)}
+                        : BandIn{B.ws + (P.xlayout == 1 ? D.off_v : D.off_vt) + ((int64_t)s * D.ny + y) * D.nxb,
+                                 1, Band(D.nxb, F.n)};
+  const int64_t o = D.off_out + line * D.nx;
+  const double cm = (B.mode && B.mode[inst] == 0) ? 0.0 : 1.0;
+  if (D.out_f32)
+    fn(F, in, SplitOut<float>{(float*)B.conc + o, (float*)B.flx + o, cm, D.px, D.nx}, !D.footprint);
+  else
+    fn(F, in, SplitOut<double>{(double*)B.conc + o, (double*)B.flx + o, cm, D.px, D.nx}, !D.footprint);
+}
+
+template <class Fn>
+__device__ __forceinline__ void run_pass(int pass, const PlanDev& P, const ExecBufs& B, const cdesc_t& D, int inst,
+                                         int64_t line, Fn&& fn) {
+  switch (pass) {
+    case PASS_FWD_ROWS: pass_fwd_rows(P, B, D, line, fn); break;
+    case PASS_FWD_COLS: pass_fwd_cols(P, B, D, line, fn); break;
+    case PASS_INV_COLS: pass_inv_cols(P, B, D, line, fn); break;
+    default: pass_inv_rows(P, B, D, inst, line, fn); break;
   }
 }
 
-// Inverse, y-pass: band column cx of slot s (compact rows scattered to their
-// padded indices), unnormalised DFT (e^{-i} footprint, e^{+i} dispersion),
-// keep the cropped rows [py, py+ny).
-template <int MAXE, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_inv_cols(PlanDev P, ExecBufs B) {
+// Single-workgroup transforms of one pass, one line per tile (XCD-swizzled
+// so that each XCD works on a contiguous range of lines).
+template <int MAXE, int MAXT, int PASS>
+__global__ __launch_bounds__(MAXT) void k_pass(PlanDev P, ExecBufs B) {
   const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t b = blockIdx.x; b < P.tot_inv_cols; b += gridDim.x) {
-    const int64_t t = gridDim.x >= P.tot_inv_cols ? xcd_swizzle(b, P.tot_inv_cols) : b;
-    const TileRec tr = tile_rec(P.t_inv_cols, t);
+  const TileRec* recs = PASS == PASS_FWD_ROWS   ? P.t_fwd_rows
+                        : PASS == PASS_FWD_COLS ? P.t_fwd_cols
+                        : PASS == PASS_INV_COLS ? P.t_inv_cols
+                                                : P.t_inv_rows;
+  const int64_t tot = PASS == PASS_FWD_ROWS   ? P.tot_fwd_rows
+                      : PASS == PASS_FWD_COLS ? P.tot_fwd_cols
+                      : PASS == PASS_INV_COLS ? P.tot_inv_cols
+                                              : P.tot_inv_rows;
+  for (int64_t b = blockIdx.x; b < tot; b += gridDim.x) {
+    const int64_t t = gridDim.x >= tot ? xcd_swizzle(b, tot) : b;
+    const TileRec tr = tile_rec(recs, t);
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
-    const int64_t local = tr.local;  // s*nxb + cx
-    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_y);
-    const BandIn in{B.ws + D.off_z + local * D.nyb, 1, Band(D.nyb, F.n)};
-    // layout 1: column cx of slot s lands at v[s][y][cx]
-    const int64_t s = local / D.nxb, cx = local % D.nxb;
-    const CropOut out = P.xlayout == 1 ? CropOut{B.ws + D.off_v + s * D.ny * D.nxb + cx, D.nxb, D.py, D.ny}
-                                       : CropOut{B.ws + D.off_v + local * D.ny, 1, D.py, D.ny};
-    fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in, out, b != blockIdx.x);
+    run_pass(PASS, P, B, D, tr.inst, tr.local, [&](const cplan_t& F, auto in, auto out, bool inv) {
+      fft_dir<MAXE>(inv, F, P.tw, tid, T, in, out, b != blockIdx.x);
+    });
   }
 }
 
-// Inverse, x-pass: cropped row y of slot s over the band columns, DFT,
-// keep [px, px+nx); conc = Re, flx = Im.
-template <int MAXE, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_inv_rows(PlanDev P, ExecBufs B) {
+// Four-step sources and sinks: step A reads the strided sub-sequence
+// in(j1 n2 + j2) and stores the twiddled length-n1 result at scr[j2 n1 + k1];
+// step B reads scr[j2 n1 + k1] over j2 and writes X[k1 + n1 k2] through the
+// pass's own sink.
+template <class In>
+struct SubIn {
+  In in;
+  int n2, j2;
+  __device__ __forceinline__ double2 operator()(int j1) const { return in(j1 * n2 + j2); }
+};
+struct TwiddleOut {
+  double2* __restrict__ dst;      // scr + j2 n1
+  const double2* __restrict__ w;  // W_n^t, t < n
+  int j2;
+  bool inv;
+  __device__ __forceinline__ void operator()(int k1, double2 v) const {
+    double2 t = w[j2 * k1];  // j2 k1 < n: no reduction modulo n
+    if (inv) t.y = -t.y;
+    dst[k1] = cmul(v, t);
+  }
+};
+struct ScrIn {
+  const double2* __restrict__ src;  // scr + k1
+  int n1;
+  __device__ __forceinline__ double2 operator()(int j2) const { return src[(int64_t)j2 * n1]; }
+};
+template <class Out>
+struct SubOut {
+  Out out;
+  int n1, k1;
+  __device__ __forceinline__ void operator()(int k2, double2 v) const { out(k1 + n1 * k2, v); }
+};
+
+// One step of the four-step transforms of the split lines of pass `pass`
+// (tile local = line * (n2 or n1) + sub-transform index).
+template <int MAXE, int MAXT, bool STEPB>
+__global__ __launch_bounds__(MAXT) void k_long(PlanDev P, ExecBufs B, int pass) {
   const int T = blockDim.x, tid = threadIdx.x;
-  for (int64_t b = blockIdx.x; b < P.tot_inv_rows; b += gridDim.x) {
-    const int64_t t = gridDim.x >= P.tot_inv_rows ? xcd_swizzle(b, P.tot_inv_rows) : b;
-    const TileRec tr = tile_rec(P.t_inv_rows, t);
+  const int k = 2 * pass + (STEPB ? 1 : 0);
+  const TileRec* recs = P.t_long[k];
+  const int64_t tot = P.tot_long[k];
+  for (int64_t b = blockIdx.x; b < tot; b += gridDim.x) {
+    const TileRec tr = tile_rec(recs, b);
     const cdesc_t& D = *(const cdesc_t*)(P.desc + tr.inst);
-    const int64_t local = tr.local;  // s*ny + y
-    const int s = (int)(local / D.ny), y = (int)(local % D.ny);
-    const cplan_t& F = *(const cplan_t*)(P.plans + D.plan_x);
-    const BandIn in = P.xlayout == 0
-                          ? BandIn{B.ws + D.off_v + (int64_t)s * D.nxb * D.ny + y, D.ny, Band(D.nxb, F.n)}
-                          : BandIn{B.ws + (P.xlayout == 1 ? D.off_v : D.off_vt) + ((int64_t)s * D.ny + y) * D.nxb,
-                                   1, Band(D.nxb, F.n)};
-    const int64_t o = D.off_out + local * D.nx;
-    // instances served by the surrogate carry no concentration spectrum
-    const double cm = (B.mode && B.mode[tr.inst] == 0) ? 0.0 : 1.0;
-    if (D.out_f32)
-      fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in,
-                    SplitOut<float>{(float*)B.conc + o, (float*)B.flx + o, cm, D.px, D.nx}, b != blockIdx.x);
-    else
-      fft_dir<MAXE>(!D.footprint, F, P.tw, tid, T, in,
-                    SplitOut<double>{(double*)B.conc + o, (double*)B.flx + o, cm, D.px, D.nx}, b != blockIdx.x);
+    const bool x_axis = pass == PASS_FWD_ROWS || pass == PASS_INV_ROWS;
+    const cplan_t& FL = *(const cplan_t*)(P.plans + (x_axis ? D.plan_x : D.plan_y));
+    const int n = FL.n, n1 = FL.n1, n2 = FL.n2;
+    const int64_t per = STEPB ? n1 : n2;
+    const int64_t line = (int64_t)tr.local / per;
+    const int sub = (int)((int64_t)tr.local % per);
+    double2* scr = B.ws + D.off_long + line * n;
+    const bool reuse = b != blockIdx.x;
+    run_pass(pass, P, B, D, tr.inst, line, [&](const cplan_t&, auto in, auto out, bool inv) {
+      if constexpr (!STEPB) {
+        const cplan_t& F1 = *(const cplan_t*)(P.plans + FL.plan1);
+        fft_dir<MAXE>(inv, F1, P.tw, tid, T, SubIn<decltype(in)>{in, n2, sub},
+                      TwiddleOut{scr + (int64_t)sub * n1, P.tw + FL.tw_off, sub, inv}, reuse);
+      } else {
+        const cplan_t& F2 = *(const cplan_t*)(P.plans + FL.plan2);
+        fft_dir<MAXE>(inv, F2, P.tw, tid, T, ScrIn{scr + sub, n1}, SubOut<decltype(out)>{out, n1, sub}, reuse);
+      }
+    });
   }
 }
 
@@ -652,13 +747,29 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
   MARK(1)
   const unsigned ft = (unsigned)P.fft_threads;
   const size_t lds = (size_t)P.fft_lds_bytes;
-#define FFT_LAUNCH(K, tiles)                                                     \
-  if (P.fft_small) K<FFT_SMALL_E, 256><<<grid_for(tiles, grid_cap), ft, lds, stream>>>(P, B); \
-  else K<BLDFM_FFT_MAXE, 1024><<<grid_for(tiles, grid_cap), ft, lds, stream>>>(P, B);        \
-  LAUNCH_CHECK();
+  // single-workgroup lines of a pass, then its split (four-step) lines
+#define FFT_LAUNCH(PASS, tiles)                                                                        \
+  if ((tiles) > 0) {                                                                                   \
+    if (P.fft_small) k_pass<FFT_SMALL_E, 256, PASS><<<grid_for(tiles, grid_cap), ft, lds, stream>>>(P, B); \
+    else k_pass<BLDFM_FFT_MAXE, 1024, PASS><<<grid_for(tiles, grid_cap), ft, lds, stream>>>(P, B);        \
+    LAUNCH_CHECK();                                                                                    \
+  }                                                                                                    \
+  for (int st_ = 0; st_ < 2; ++st_) {                                                                  \
+    const int64_t tl_ = P.tot_long[2 * (PASS) + st_];                                                  \
+    if (tl_ <= 0) continue;                                                                            \
+    const unsigned g_ = grid_for(tl_, grid_cap);                                                       \
+    if (P.fft_small) {                                                                                 \
+      if (st_) k_long<FFT_SMALL_E, 256, true><<<g_, ft, lds, stream>>>(P, B, PASS);                    \
+      else k_long<FFT_SMALL_E, 256, false><<<g_, ft, lds, stream>>>(P, B, PASS);                       \
+    } else {                                                                                           \
+      if (st_) k_long<BLDFM_FFT_MAXE, 1024, true><<<g_, ft, lds, stream>>>(P, B, PASS);                \
+      else k_long<BLDFM_FFT_MAXE, 1024, false><<<g_, ft, lds, stream>>>(P, B, PASS);                   \
+    }                                                                                                  \
+    LAUNCH_CHECK();                                                                                    \
+  }
   if (P.any_dispersion) {
-    FFT_LAUNCH(k_fwd_rows, P.tot_fwd_rows)
-    FFT_LAUNCH(k_fwd_cols, P.tot_fwd_cols)
+    FFT_LAUNCH(PASS_FWD_ROWS, P.tot_fwd_rows)
+    FFT_LAUNCH(PASS_FWD_COLS, P.tot_fwd_cols)
   }
   MARK(2)
   const bool sur = P.sur_net != nullptr && B.mode != nullptr && P.tot_sur > 0;
@@ -705,13 +816,13 @@ int bldfm_launch_all(const PlanDev& P, const ExecBufs& B, int grid_cap, hipStrea
     LAUNCH_CHECK();
   }
   MARK(5)
-  FFT_LAUNCH(k_inv_cols, P.tot_inv_cols)
+  FFT_LAUNCH(PASS_INV_COLS, P.tot_inv_cols)
   MARK(6)
   if (P.xlayout == 2 && P.tot_tr > 0) {
     k_transpose_v<<<grid_for(P.tot_tr, grid_cap), 256, 0, stream>>>(P, B);
     LAUNCH_CHECK();
   }
-  FFT_LAUNCH(k_inv_rows, P.tot_inv_rows)
+  FFT_LAUNCH(PASS_INV_ROWS, P.tot_inv_rows)
   MARK(7)
 #undef FFT_LAUNCH
 #undef MARK
@@ -782,35 +893,95 @@ __global__ __launch_bounds__(MAXT) void k_fft2_cols(const FftPlan* F, const doub
   }
 }
 
-int bldfm_launch_fft2(const FftPlan* F0, int32_t n0, const double2* tw0, const FftPlan* F1, int32_t n1,
-                      const double2* tw1, int32_t batch, const double2* in, double2* mid, double2* out,
-                      int32_t sign, double scale, int threads, int lds_bytes, int grid_cap,
-                      hipStream_t stream) {
+// Split (four-step) lines of the plain 2-D FFT: `rows` selects the rows of
+// the (n0, n1) arrays (line r at r*n1, stride 1), else their columns (line c
+// at (c / n1)*n0*n1 + c % n1, stride n1); scr holds nlines x n values.
+template <int MAXE, int MAXT, bool STEPB>
+__global__ __launch_bounds__(MAXT) void k_fft2_long(FftSplit S, const double2* __restrict__ in,
+                                                  double2* __restrict__ out, double2* __restrict__ scr, int rows,
+                                                  int32_t n0, int32_t n1, int64_t nlines, double scale, bool inv) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int n = S.F->n, s1 = S.F->n1, s2 = S.F->n2;
+  const int64_t per = STEPB ? s1 : s2;
+  for (int64_t b = blockIdx.x; b < nlines * per; b += gridDim.x) {
+    const int64_t line = b / per;
+    const int sub = (int)(b % per);
+    const int64_t base = rows ? line * n1 : (line / n1) * (int64_t)n0 * n1 + line % n1;
+    const int64_t stride = rows ? 1 : n1;
+    double2* sc = scr + line * n;
+    const bldfm_fft::Lds buf0 = bldfm_fft::lds_view(smem, STEPB ? s2 : s1);
+    if constexpr (!STEPB) {
+      const SubIn<StridedIn> src{StridedIn{in + base, stride}, s2, sub};
+      const TwiddleOut dst{sc + (int64_t)sub * s1, S.tw, sub, inv};
+      if (inv) bldfm_fft::lds_fft<MAXE, true>(buf0, S.F1, S.tw1, tid, T, src, dst, b != blockIdx.x);
+      else bldfm_fft::lds_fft<MAXE, false>(buf0, S.F1, S.tw1, tid, T, src, dst, b != blockIdx.x);
+    } else {
+      const ScrIn src{sc + sub, s1};
+      const SubOut<StridedOut> dst{StridedOut{out + base, stride, scale}, s1, sub};
+      if (inv) bldfm_fft::lds_fft<MAXE, true>(buf0, S.F2, S.tw2, tid, T, src, dst, b != blockIdx.x);
+      else bldfm_fft::lds_fft<MAXE, false>(buf0, S.F2, S.tw2, tid, T, src, dst, b != blockIdx.x);
+    }
+  }
+}
+
+int bldfm_launch_fft2(const FftSplit& A0, int32_t n0, const FftSplit& A1, int32_t n1, int32_t batch,
+                      const double2* in, double2* mid, double2* scr, double2* out, int32_t sign, double scale,
+                      int threads, int lds_bytes, int grid_cap, hipStream_t stream) {
   const bool inv = sign > 0;
   const int64_t nrows = (int64_t)batch * n0, ncols = (int64_t)batch * n1;
-  if (threads <= 256) {
-    k_fft2_rows<FFT_SMALL_E, 256><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, n1, nrows, inv);
-    LAUNCH_CHECK();
-    k_fft2_cols<FFT_SMALL_E, 256><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, n0, n1, ncols, scale, inv);
-    LAUNCH_CHECK();
+  const bool small = threads <= 256;
+  // rows (axis 1, length n1): in -> mid
+  if (A1.F1) {
+    const unsigned ga = grid_for(nrows * A1.n2, grid_cap), gb = grid_for(nrows * A1.n1, grid_cap);
+    if (small) {
+      k_fft2_long<FFT_SMALL_E, 256, false><<<ga, threads, lds_bytes, stream>>>(A1, in, nullptr, scr, 1, n0, n1, nrows, 1.0, inv);
+      LAUNCH_CHECK();
+      k_fft2_long<FFT_SMALL_E, 256, true><<<gb, threads, lds_bytes, stream>>>(A1, nullptr, mid, scr, 1, n0, n1, nrows, 1.0, inv);
+    } else {
+      k_fft2_long<BLDFM_FFT_MAXE, 1024, false><<<ga, threads, lds_bytes, stream>>>(A1, in, nullptr, scr, 1, n0, n1, nrows, 1.0, inv);
+      LAUNCH_CHECK();
+      k_fft2_long<BLDFM_FFT_MAXE, 1024, true><<<gb, threads, lds_bytes, stream>>>(A1, nullptr, mid, scr, 1, n0, n1, nrows, 1.0, inv);
+    }
+  } else if (small) {
+    k_fft2_rows<FFT_SMALL_E, 256><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(A1.F, A1.tw, in, mid, n1, nrows, inv);
   } else {
-    k_fft2_rows<BLDFM_FFT_MAXE, 1024><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(F1, tw1, in, mid, n1, nrows, inv);
-    LAUNCH_CHECK();
-    k_fft2_cols<BLDFM_FFT_MAXE, 1024><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(F0, tw0, mid, out, n0, n1, ncols, scale, inv);
-    LAUNCH_CHECK();
+    k_fft2_rows<BLDFM_FFT_MAXE, 1024><<<grid_for(nrows, grid_cap), threads, lds_bytes, stream>>>(A1.F, A1.tw, in, mid, n1, nrows, inv);
   }
+  LAUNCH_CHECK();
+  // columns (axis 0, length n0): mid -> out, scaled
+  if (A0.F1) {
+    const unsigned ga = grid_for(ncols * A0.n2, grid_cap), gb = grid_for(ncols * A0.n1, grid_cap);
+    if (small) {
+      k_fft2_long<FFT_SMALL_E, 256, false><<<ga, threads, lds_bytes, stream>>>(A0, mid, nullptr, scr, 0, n0, n1, ncols, 1.0, inv);
+      LAUNCH_CHECK();
+      k_fft2_long<FFT_SMALL_E, 256, true><<<gb, threads, lds_bytes, stream>>>(A0, nullptr, out, scr, 0, n0, n1, ncols, scale, inv);
+    } else {
+      k_fft2_long<BLDFM_FFT_MAXE, 1024, false><<<ga, threads, lds_bytes, stream>>>(A0, mid, nullptr, scr, 0, n0, n1, ncols, 1.0, inv);
+      LAUNCH_CHECK();
+      k_fft2_long<BLDFM_FFT_MAXE, 1024, true><<<gb, threads, lds_bytes, stream>>>(A0, nullptr, out, scr, 0, n0, n1, ncols, scale, inv);
+    }
+  } else if (small) {
+    k_fft2_cols<FFT_SMALL_E, 256><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(A0.F, A0.tw, mid, out, n0, n1, ncols, scale, inv);
+  } else {
+    k_fft2_cols<BLDFM_FFT_MAXE, 1024><<<grid_for(ncols, grid_cap), threads, lds_bytes, stream>>>(A0.F, A0.tw, mid, out, n0, n1, ncols, scale, inv);
+  }
+  LAUNCH_CHECK();
   return 0;
 }
 
 // Allow dynamic LDS beyond the default per-workgroup cap for the FFT kernels.
 void bldfm_set_fft_lds_attr(int bytes) {
   const void* fns[] = {
-      (const void*)k_fwd_rows<FFT_SMALL_E, 256>, (const void*)k_fwd_cols<FFT_SMALL_E, 256>,
-      (const void*)k_inv_cols<FFT_SMALL_E, 256>, (const void*)k_inv_rows<FFT_SMALL_E, 256>,
+      (const void*)k_pass<FFT_SMALL_E, 256, PASS_FWD_ROWS>, (const void*)k_pass<FFT_SMALL_E, 256, PASS_FWD_COLS>,
+      (const void*)k_pass<FFT_SMALL_E, 256, PASS_INV_COLS>, (const void*)k_pass<FFT_SMALL_E, 256, PASS_INV_ROWS>,
+      (const void*)k_long<FFT_SMALL_E, 256, false>, (const void*)k_long<FFT_SMALL_E, 256, true>,
       (const void*)k_fft2_rows<FFT_SMALL_E, 256>, (const void*)k_fft2_cols<FFT_SMALL_E, 256>,
-      (const void*)k_fwd_rows<BLDFM_FFT_MAXE, 1024>, (const void*)k_fwd_cols<BLDFM_FFT_MAXE, 1024>,
-      (const void*)k_inv_cols<BLDFM_FFT_MAXE, 1024>, (const void*)k_inv_rows<BLDFM_FFT_MAXE, 1024>,
-      (const void*)k_fft2_rows<BLDFM_FFT_MAXE, 1024>, (const void*)k_fft2_cols<BLDFM_FFT_MAXE, 1024>};
+      (const void*)k_fft2_long<FFT_SMALL_E, 256, false>, (const void*)k_fft2_long<FFT_SMALL_E, 256, true>,
+      (const void*)k_pass<BLDFM_FFT_MAXE, 1024, PASS_FWD_ROWS>, (const void*)k_pass<BLDFM_FFT_MAXE, 1024, PASS_FWD_COLS>,
+      (const void*)k_pass<BLDFM_FFT_MAXE, 1024, PASS_INV_COLS>, (const void*)k_pass<BLDFM_FFT_MAXE, 1024, PASS_INV_ROWS>,
+      (const void*)k_long<BLDFM_FFT_MAXE, 1024, false>, (const void*)k_long<BLDFM_FFT_MAXE, 1024, true>,
+      (const void*)k_fft2_rows<BLDFM_FFT_MAXE, 1024>, (const void*)k_fft2_cols<BLDFM_FFT_MAXE, 1024>,
+      (const void*)k_fft2_long<BLDFM_FFT_MAXE, 1024, false>, (const void*)k_fft2_long<BLDFM_FFT_MAXE, 1024, true>};
   for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -34,6 +34,12 @@
 // Ns = radix[0]*...*radix[s-1]; stride[s] = n/(Ns*radix[s]) is its twiddle
 // index step and magic[s] = floor(2^32/Ns) + 1 divides by Ns with one
 // multiply-high (j/Ns = (j*magic) >> 32 for j < 2^16, Ns >= 2; 0 at s = 0).
+//
+// A length too long for one workgroup's LDS is split, n = n1 * n2 (the
+// four-step method): n2 transforms of length n1 over the strided inputs
+// x[j1 n2 + j2], the twiddle W_n^{j2 k1}, then n1 transforms of length n2
+// giving X[k1 + n1 k2].  Such a plan has nstage = 0, n1 > 0 and the plan ids
+// (or, for the plain 2-D FFT, unused) of its two factor lengths.
 struct FftPlan {
   int32_t n;
   int32_t nstage;
@@ -41,7 +47,22 @@ struct FftPlan {
   int32_t stride[BLDFM_MAX_RADIX_STAGES];
   uint32_t magic[BLDFM_MAX_RADIX_STAGES];
   int64_t tw_off;  // offset (in double2) of the n twiddles e^{-2 pi i t/n}
+  int32_t n1, n2;        // four-step factors (n1 = 0: one LDS transform)
+  int32_t plan1, plan2;  // plan ids of the factor lengths
 };
+
+// Factor n1 of a four-step split of a length n > limit: the smallest divisor
+// of n at or above sqrt(n) with n1 <= limit (then n2 = n/n1 <= sqrt(n) <=
+// limit); 0 when n fits one workgroup, -1 when no such split exists (n has
+// no divisor in [n/limit, limit], e.g. a prime).
+BLDFM_HD int fft_split(int n, int limit) {
+  if (n <= limit) return 0;
+  int r = 1;
+  while ((int64_t)r * r < n) ++r;
+  for (int d = r; d <= limit; ++d)
+    if (n % d == 0 && n / d <= limit) return d;
+  return -1;
+}
 
 // Twiddle tables of an LDS transform of length n: W^lo (lo < L = 2^lb,
 // L*L >= n) and W^{hi*L} (hi < ceil(n/L)), behind the n-point buffer.
@@ -110,6 +131,8 @@ struct InstDesc {
                         // when the plan's x-pass layout is 1
   int64_t off_vt;       // workspace double2: (nslot, ny, nxb) transpose of v (x-pass layout 2)
   int64_t off_out;      // output element offset: (nslot, ny, nx)
+  int64_t off_long;     // workspace double2: four-step scratch (lines x n of the longest split pass)
+  int32_t long_x, long_y;  // 1: the padded x / y length is split (four-step passes)
 };
 
 // ---------------------------------------------------------------- frequencies

@@ -114,6 +114,30 @@ int main() {
       check_case(nly, a, nly + 4, a);
       cases += 2;
     }
+  // four-step splits of long transforms: n1 * n2 = n, both within the limit,
+  // n1 the smallest divisor at or above sqrt(n); -1 exactly when no divisor
+  // pair fits
+  const int limit = 10033;
+  for (int n = 1; n <= 300000; ++n) {
+    const int s = fft_split(n, limit);
+    if (n <= limit) {
+      CHECK(s == 0, "n %d fits one workgroup but split %d", n, s);
+      continue;
+    }
+    bool any = false;
+    int best = -1;
+    for (int d = 2; d <= limit; ++d)
+      if (n % d == 0 && n / d <= limit) {
+        any = true;
+        if ((int64_t)d * d >= n && best < 0) best = d;
+      }
+    if (!any) {
+      CHECK(s == -1, "n %d has no split but got %d", n, s);
+    } else {
+      CHECK(s > 0 && n % s == 0 && s <= limit && n / s <= limit && s == best, "n %d split %d (best %d)", n, s, best);
+    }
+  }
+  ++cases;
   // production shapes
   check_case(512, 512, 1536, 1536);
   check_case(64, 128, 960, 384);

@@ -284,26 +284,71 @@ def test_fft2(gpu, shape, norm):
         assert rel_l2(got.real, exp.real) <= 1e-13 and rel_l2(got.imag, exp.imag) <= 1e-13
 
 
+@pytest.mark.parametrize("shape", [(3, 12288), (12288, 2), (2, 10200, 3), (10201, 4)])
+def test_fft2_four_step(gpu, shape):
+    """Axes longer than one workgroup's LDS transform (10033 points) run as
+    four-step transforms n = n1 n2 (12288 = 128 x 96, 10200 = 102 x 100,
+    10201 = 101 x 101): numpy fft2/ifft2 at 1e-13 in every norm."""
+    from bldfm_amd import fft_manager
+
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    for norm in ("backward", "forward", "ortho"):
+        for ours, ref in ((fft_manager.fft2, np.fft.fft2), (fft_manager.ifft2, np.fft.ifft2)):
+            got, exp = ours(x, norm=norm), ref(x, norm=norm)
+            assert rel_l2(got.real, exp.real) <= 1e-13 and rel_l2(got.imag, exp.imag) <= 1e-13, (norm, ours)
+
+
 def test_fft2_length_limit(gpu):
-    """A transform that cannot hold its buffer and twiddle tables in one
-    workgroup's LDS is refused with a clear error, not run."""
+    """A length above the single-workgroup limit without a factor pair within
+    it (the prime 10037) is refused with a clear error, not run."""
     from bldfm_amd import _lib, fft_manager
 
     with pytest.raises((ValueError, _lib.BLDFMError), match="single-workgroup FFT limit"):
-        fft_manager.fft2(np.ones((2, 10240), dtype=complex))
+        fft_manager.fft2(np.ones((2, 10037), dtype=complex))
 
 
-def test_solver_length_limit(gpu):
-    """A solve whose padded grid exceeds the single-workgroup FFT limit
-    (3400 cells with the default halo: 10200 points) is refused with the
-    same clear error, before any device buffer is allocated."""
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("footprint", [True, False])
+def test_solver_four_step_axis(gpu, axis, footprint):
+    """A 4096-cell axis with the default halo pads to 12288 points (above the
+    single-workgroup limit): its transforms run as four-step transforms.
+    Both directions of the path are exercised -- footprint (inverse passes
+    only) and dispersion (forward passes too) -- with the long axis along x
+    and along y; the other axis is kept short so the oracle stays cheap.
+    FP64 parity with the oracle at rel-L2 <= 1e-10."""
+    _, solver = gpu
+    from bldfm_amd.utils import ideal_source
+
+    z, profs = opbl.vertical_profiles(12, 10.0, (3.0, 1.0), ustar=0.3, mol=-40.0)
+    nxy = (4096, 8) if axis == "x" else (8, 4096)
+    modes = (1024, 16) if axis == "x" else (16, 1024)
+    domain = (4096.0, 4096.0)  # default halo 4096 m: 12288 points on the 1 m axis, 24 on the other
+    if footprint:
+        q0, levels, kw = np.zeros(nxy[::-1]), 12, dict(meas_pt=(2048.0, 2048.0), footprint=True)
+    else:
+        q0, levels, kw = ideal_source(nxy, domain, shape="diamond"), [4, 12], dict()
+    print(f"[four-step] axis {axis} footprint={footprint}", flush=True)
+    _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, domain, levels, modes=modes,
+                                                     precision="double", **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, domain, levels, modes=modes,
+                                                      precision="double", **kw)
+    print(f"  conc {rel_l2(gc, oc):.2e} flx {rel_l2(gf, of):.2e}")
+    assert np.abs(gf).max() > 0
+    assert rel_l2(gc, oc) <= 1e-10 and rel_l2(gf, of) <= 1e-10
+
+
+def test_solver_length_without_split_is_refused(gpu):
+    """A padded length above the limit with no factor pair within it
+    (20074 = 2 x 10037 = 3346 cells + 2 x 8364 halo cells) is refused with
+    the clear error before any device buffer is allocated."""
     from bldfm_amd import _lib
 
     _, solver = gpu
     z, profs = opbl.vertical_profiles(8, 10.0, (3.0, 1.0), ustar=0.3, mol=-40.0)
     with pytest.raises((ValueError, _lib.BLDFMError), match="single-workgroup FFT limit"):
-        solver.steady_state_transport_solver(np.zeros((16, 3400)), z, profs, (3400.0, 16.0), 8, modes=(64, 16),
-                                             meas_pt=(1700.0, 8.0), footprint=True)
+        solver.steady_state_transport_solver(np.zeros((16, 3346)), z, profs, (3346.0, 16.0), 8, modes=(64, 16),
+                                             halo=8364.0, meas_pt=(1700.0, 8.0), footprint=True)
 
 
 def test_device_math_rounds_like_host(gpu):
