@@ -108,7 +108,10 @@ def _instance(spec: FootprintSpec, precision: str, fast: bool, surrogate: bool =
             slot[i] = k
             k += 1
     idx = np.mod(np.asarray(levels, dtype=np.int64), inst.nz).astype(np.int32)
-    return inst, np.concatenate([slot, idx])
+    from .solver import out_extent
+
+    dims = (out_extent(ny, py, nly, nye), out_extent(nx, px, nlx, nxe))
+    return inst, np.concatenate([slot, idx]), dims
 
 
 class FootprintBatch:
@@ -136,6 +139,7 @@ class FootprintBatch:
         mets = _lib.make_mets(self.specs)
         built = [_instance(s, precision, fast, surrogate is not None, m) for s, m in zip(self.specs, mets)]
         self.instances = [b[0] for b in built]
+        self.out_dims = [b[2] for b in built]  # (rows, columns) of each instance's outputs
         ints = np.concatenate([b[1] for b in built]).astype(np.int32)
         self.levels_i32 = ints
         self.tune_report = None
@@ -219,8 +223,9 @@ class FootprintBatch:
         """(conc, flx) device views of instance i, shape (nslot, ny, nx)."""
         _, _, off, _ = self.plan.offsets[i]
         s = self.specs[i]
-        n = self.instances[i].nslot * s.ny * s.nx
-        shape = (self.instances[i].nslot, s.ny, s.nx)
+        ny, nx = self.out_dims[i]
+        n = self.instances[i].nslot * ny * nx
+        shape = (self.instances[i].nslot, ny, nx)
         return self.conc[off : off + n].view(shape), self.flx[off : off + n].view(shape)
 
     def profiles(self, i):

@@ -410,17 +410,27 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     D.nxe = I.nx + 2 * I.px;
     D.nye = I.ny + 2 * I.py;
     if (D.nlx > D.nxe || D.nly > D.nye) return bad(BLDFM_EINVAL, "modes exceed the padded grid");
-    if ((D.nxe - D.nlx) % 2 || (D.nye - D.nly) % 2)
+    // Transform lengths: the reference's untruncation pads the nl retained
+    // modes by (ne - nl)//2 on each side (src/bldfm/solver.py:265-278), so with
+    // an odd ne - nl a footprint's inverse transform has ne - 1 points, and its
+    // crop [p, ne - p) keeps min(n, ne - 1 - p) of them; a dispersion solve
+    // fails there in the reference (its truncated source spectrum carries
+    // nl + 1 modes, solver.py:138-145), so it is refused.
+    const int Lx = D.nlx + 2 * ((D.nxe - D.nlx) / 2), Ly = D.nly + 2 * ((D.nye - D.nly) / 2);
+    if (!I.footprint && (Lx != D.nxe || Ly != D.nye))
       return bad(BLDFM_EINVAL, "padded grid and mode counts must have equal parity.");
+    D.nx = std::min(I.nx, Lx - I.px);  // output columns and rows (n unless ne - nl is odd)
+    D.ny = std::min(I.ny, Ly - I.py);
+    if (D.nx <= 0 || D.ny <= 0) return bad(BLDFM_EINVAL, "the crop of the inverse transform is empty");
     int split_x = 0, split_y = 0;
-    int rc = check_fft_len(D.nxe, device, "padded x length", &split_x);
-    if (!rc) rc = check_fft_len(D.nye, device, "padded y length", &split_y);
+    int rc = check_fft_len(Lx, device, "padded x length", &split_x);
+    if (!rc) rc = check_fft_len(Ly, device, "padded y length", &split_y);
     if (rc) { delete P; return rc; }
     D.long_x = split_x > 0;
     D.long_y = split_y > 0;
-    max_n = std::max(max_n, std::max(lds_len(D.nxe, split_x), lds_len(D.nye, split_y)));
-    D.nxb = band_len(D.nlx, D.nxe);
-    D.nyb = band_len(D.nly, D.nye);
+    max_n = std::max(max_n, std::max(lds_len(Lx, split_x), lds_len(Ly, split_y)));
+    D.nxb = band_len(D.nlx, Lx);
+    D.nyb = band_len(D.nly, Ly);
     if (I.prof_from_met) {
       rc = met_setup(&I.met, &met[i]);
       if (rc) { delete P; return rc; }
@@ -439,10 +449,10 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     if (out_f32 < 0) out_f32 = D.out_f32;
     if (out_f32 != D.out_f32) return bad(BLDFM_EINVAL, "all instances of a plan must share out_f32");
     if (!D.footprint) any_disp = 1;
-    D.plan_x = plan_id(D.nxe);
-    D.plan_y = plan_id(D.nye);
-    if (split_x) { plan_id(split_x); plan_id(D.nxe / split_x); }
-    if (split_y) { plan_id(split_y); plan_id(D.nye / split_y); }
+    D.plan_x = plan_id(Lx);
+    D.plan_y = plan_id(Ly);
+    if (split_x) { plan_id(split_x); plan_id(Lx / split_x); }
+    if (split_y) { plan_id(split_y); plan_id(Ly / split_y); }
     D.lanes = shoot_lanes(D.nlx, D.nly);
     D.dx = I.dx; D.dy = I.dy;
     if (!(I.dx > 0.0) || !(I.dy > 0.0)) return bad(BLDFM_EINVAL, "grid increments must be positive");
@@ -506,12 +516,12 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     // four-step scratch: lines x n of the largest split pass
     int64_t long_elems = 0;
     if (D.long_x) {
-      long_elems = std::max(long_elems, (int64_t)D.nslot * D.ny * D.nxe);
-      if (!D.footprint) long_elems = std::max(long_elems, (int64_t)D.ny * D.nxe);
+      long_elems = std::max(long_elems, (int64_t)D.nslot * D.ny * Lx);
+      if (!D.footprint) long_elems = std::max(long_elems, (int64_t)D.ny * Lx);
     }
     if (D.long_y) {
-      long_elems = std::max(long_elems, (int64_t)D.nslot * D.nxb * D.nye);
-      if (!D.footprint) long_elems = std::max(long_elems, (int64_t)D.nlx * D.nye);
+      long_elems = std::max(long_elems, (int64_t)D.nslot * D.nxb * Ly);
+      if (!D.footprint) long_elems = std::max(long_elems, (int64_t)D.nlx * Ly);
     }
     D.off_long = long_elems ? off_ws : -1;
     off_ws += long_elems;
@@ -522,8 +532,8 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     // lines of each pass; a split axis moves them to the four-step tiles
     const int64_t lines_fr = D.footprint ? 0 : D.ny, lines_fc = D.footprint ? 0 : D.nlx;
     const int64_t lines_ic = (int64_t)D.nslot * D.nxb, lines_ir = (int64_t)D.nslot * D.ny;
-    const int64_t x1 = split_x, x2 = split_x ? D.nxe / split_x : 0;
-    const int64_t y1 = split_y, y2 = split_y ? D.nye / split_y : 0;
+    const int64_t x1 = split_x, x2 = split_x ? Lx / split_x : 0;
+    const int64_t y1 = split_y, y2 = split_y ? Ly / split_y : 0;
     const int64_t nt[kKinds] = {
         (D.nz + 255) / 256,
         D.long_x ? 0 : lines_fr,

@@ -96,7 +96,8 @@ enum LaneKind : int32_t {
 // buffers named in the comments.
 struct InstDesc {
   // geometry
-  int32_t nx, ny;      // surface grid, srf_flx is (ny, nx)
+  int32_t nx, ny;      // output grid (nslot, ny, nx); srf_flx is (ny, nx) (the same except for
+                       // a footprint whose padded length and mode count differ in parity)
   int32_t nlx, nly;    // retained modes (after clamping)
   int32_t nxe, nye;    // padded grid
   int32_t px, py;      // halo pad widths

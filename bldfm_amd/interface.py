@@ -141,8 +141,8 @@ class _PairPlan:
         for k, ((tower, i), g) in enumerate(zip(self.pairs, self.geo)):
             op, _, oo, nz = p.offsets[k]
             z = prof_h[op : op + nz]
-            n = len(g["levels"]) * g["ny"] * g["nx"]
-            shape = (len(g["levels"]), g["ny"], g["nx"])
+            n = len(g["levels"]) * g["nyo"] * g["nxo"]
+            shape = (len(g["levels"]), g["nyo"], g["nxo"])
             c = conc_h[oo : oo + n].reshape(shape)
             f = flx_h[oo : oo + n].reshape(shape)
             x = np.linspace(0, dom.xmax, g["nx"], endpoint=False)

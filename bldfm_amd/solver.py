@@ -43,6 +43,14 @@ def _level_maps(levels, nz):
     return slot, idx
 
 
+def out_extent(n, p, nl, ne):
+    """Output points along an axis: the reference pads the nl retained modes
+    by (ne - nl)//2 on each side (src/bldfm/solver.py:265-278), so the inverse
+    transform has nl + 2*((ne - nl)//2) points -- ne, or ne - 1 when ne - nl is
+    odd (footprint mode) -- and the crop [p, ne - p) of it keeps this many."""
+    return min(ne - p, nl + 2 * ((ne - nl) // 2)) - p
+
+
 def _geometry(srf_flx_shape, z, domain, levels, modes, meas_pt, footprint, halo, precision):
     """Reference argument handling, src/bldfm/solver.py:82-130, 177-188."""
     xmx, ymx = domain
@@ -65,11 +73,15 @@ def _geometry(srf_flx_shape, z, domain, levels, modes, meas_pt, footprint, halo,
         nlx, nly = nxe, nye
     if precision not in ("single", "double"):
         raise ValueError("precision must be single (default) or double.")
-    if (nxe - nlx) % 2 or (nye - nly) % 2:
-        raise ValueError(
-            "padded grid and mode counts must have equal parity "
-            f"(padded {nye}x{nxe}, modes {nly}x{nlx})."
+    if not footprint and ((nxe - nlx) % 2 or (nye - nly) % 2):
+        # the truncated source spectrum keeps nl + 1 modes on such an axis and
+        # the reference's boolean mode mask fails on it (src/bldfm/solver.py:138-145, 226)
+        ax, nl = (0, nly) if (nye - nly) % 2 else (1, nlx)
+        raise IndexError(
+            f"boolean index did not match indexed array along axis {ax}; "
+            f"size of axis is {nl + 1} but size of corresponding boolean axis is {nl}"
         )
+    nxo, nyo = out_extent(nx, px, nlx, nxe), out_extent(ny, py, nly, nye)
     if footprint:
         shift = (xm + halo, ym + halo)
         has_shift = True
@@ -79,7 +91,7 @@ def _geometry(srf_flx_shape, z, domain, levels, modes, meas_pt, footprint, halo,
     else:
         shift, has_shift = (0.0, 0.0), False
     return dict(
-        nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, px=px, py=py, nlx=nlx, nly=nly,
+        nx=nx, ny=ny, nxo=nxo, nyo=nyo, nz=nz, dx=dx, dy=dy, px=px, py=py, nlx=nlx, nly=nly,
         levels=levels, halo=halo, shift=shift, has_shift=has_shift,
         out_f32=(precision == "single" and not has_shift),
     )
@@ -172,7 +184,7 @@ def solve_device(
         src = srf_flx if torch.is_tensor(srf_flx) else torch.as_tensor(np.ascontiguousarray(srf_flx, dtype=np.float64))
         src = src.to(device=dev, dtype=torch.float64).contiguous()
     odt = torch.float32 if g["out_f32"] else torch.float64
-    conc = torch.empty((nslot, g["ny"], g["nx"]), dtype=odt, device=dev)
+    conc = torch.empty((nslot, g["nyo"], g["nxo"]), dtype=odt, device=dev)
     flx = torch.empty_like(conc)
     ws = torch.empty(max(plan.workspace_bytes, 1), dtype=torch.uint8, device=dev)
     plan.execute(prof, src, conc, flx, ws)

@@ -162,19 +162,26 @@ def steady_state_transport_solver(
     nxe, nye = nx + 2 * px, ny + 2 * py
     if (nlx > nxe) or (nly > nye):
         nlx, nly = nxe, nye
-    if (nxe - nlx) % 2 or (nye - nly) % 2:
-        # The reference's fftshift/slice/pad sequence breaks for this parity.
-        raise ValueError("padded grid and mode counts must have equal parity.")
-
-    iy = padded_index(nly, nye)
-    ix = padded_index(nlx, nxe)
+    # The untruncation pads the nl retained modes by (ne - nl)//2 on each side
+    # (solver.py:265-278), so the inverse transforms have lx_t = nl + 2 dl
+    # points: ne, or ne - 1 when ne - nl is odd.
+    dlx, dly = (nxe - nlx) // 2, (nye - nly) // 2
+    ltx, lty = nlx + 2 * dlx, nly + 2 * dly
 
     if footprint:
         tq0 = np.ones((nly, nlx), dtype=np.complex128) / nxe / nye
     else:
         qpad = np.zeros((nye, nxe))
         qpad[py : py + ny, px : px + nx] = q0
-        tq0 = np.fft.fft2(qpad, norm="forward")[np.ix_(iy, ix)]
+        fq = np.fft.fft2(qpad, norm="forward")
+        if ltx == nxe and lty == nye:
+            tq0 = fq[np.ix_(padded_index(nly, nye), padded_index(nlx, nxe))]
+        else:
+            # odd ne - nl: the reference's fftshift/slice/ifftshift keeps
+            # nl + 1 modes (solver.py:138-145); its mode mask below fails on it
+            tq0 = np.fft.ifftshift(np.fft.fftshift(fq)[dly : nye - dly, dlx : nxe - dlx])
+    iy = padded_index(nly, lty)
+    ix = padded_index(nlx, ltx)
 
     lx = wavenumbers(nlx, dx, nxe)
     ly = wavenumbers(nly, dy, nye)
@@ -204,7 +211,7 @@ def steady_state_transport_solver(
     if analytic:
         h = z[levels] - z[0]
         tp[:, 0, 0] = srf_bg_conc - tq0[0, 0] * kinv_t * h
-        tq[:, msk] = tq0[msk] * np.exp(-eig * h)
+        tq[:, msk] = tq0[msk] * np.exp(-eig * h)  # IndexError for odd ne - nl (solver.py:197)
         tp[:, msk] = tq[:, msk] * kinv_t / eig
     else:
         one = np.ones(msk.sum(), dtype=np.complex128)
@@ -241,8 +248,8 @@ def steady_state_transport_solver(
         shift = np.exp(1j * (Lx * (xm - xmx / 2) + Ly * (ym - ymx / 2)))
         tp, tq = tp * shift, tq * shift
 
-    P = np.zeros((nlvls, nye, nxe), dtype=tp.dtype)
-    Q = np.zeros((nlvls, nye, nxe), dtype=tq.dtype)
+    P = np.zeros((nlvls, lty, ltx), dtype=tp.dtype)
+    Q = np.zeros((nlvls, lty, ltx), dtype=tq.dtype)
     P[:, iy[:, None], ix[None, :]] = tp
     Q[:, iy[:, None], ix[None, :]] = tq
     if footprint:

@@ -351,6 +351,27 @@ def test_solver_length_without_split_is_refused(gpu):
                                              halo=8364.0, meas_pt=(1700.0, 8.0), footprint=True)
 
 
+@pytest.mark.parametrize("nx,ny,modes,halo", [(129, 64, (64, 32), None), (64, 129, (32, 64), None),
+                                              (129, 127, (64, 32), None), (129, 64, (64, 32), 0.0)])
+def test_odd_padded_length_footprint(gpu, nx, ny, modes, halo):
+    """Footprints whose padded length and mode count differ in parity run as
+    in the reference: inverse transforms of ne - 1 points, crop of
+    min(n, ne - 1 - p) points (with no halo one column or row short of the
+    grid).  rel-L2 <= 1e-10 against the oracle; a dispersion solve of the
+    same geometry raises the reference's IndexError before any device work."""
+    _, solver = gpu
+    z, profs = opbl.vertical_profiles(16, 10.0, (3.0, 1.0), ustar=0.3, mol=-40.0)
+    domain = (float(2 * nx), float(2 * ny))
+    kw = dict(modes=modes, meas_pt=(nx * 1.0, ny * 1.0), footprint=True, precision="double", halo=halo)
+    _, gc, gf = solver.steady_state_transport_solver(np.zeros((ny, nx)), z, profs, domain, 16, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(np.zeros((ny, nx)), z, profs, domain, 16, **kw)
+    print(f"  {nx}x{ny} modes {modes} halo {halo}: shape {gf.shape}, flx {rel_l2(gf, of):.2e}")
+    assert gc.shape == oc.shape and gf.shape == of.shape
+    assert rel_l2(gc, oc) <= 1e-10 and rel_l2(gf, of) <= 1e-10
+    with pytest.raises(IndexError, match="boolean index did not match"):
+        solver.steady_state_transport_solver(np.ones((ny, nx)), z, profs, domain, 16, modes=modes, halo=halo)
+
+
 def test_device_math_rounds_like_host(gpu):
     """Device hypot / complex sqrt / divide / multiply round exactly like the
     host's numpy on the value ranges the solver feeds them."""

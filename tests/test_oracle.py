@@ -224,3 +224,30 @@ def test_hermitian_symmetry_of_the_spectrum():
                 a = X[jy, jx]
                 b = X[(-f[jy]) % 32, (-f[jx]) % 32]
                 assert abs(a - np.conj(b)) <= 1e-12 * abs(X).max()
+
+
+@pytest.mark.parametrize("nx,ny,modes,halo", [(33, 20, (16, 12), 9.0), (20, 33, (12, 16), 9.0),
+                                              (33, 21, (16, 12), 9.0), (33, 20, (16, 12), 0.0)])
+def test_odd_padded_length_footprint_follows_the_reference_sequence(nx, ny, modes, halo):
+    """With an odd ne - nl the reference still runs a footprint: its
+    fftshift / pad by (ne - nl)//2 / ifftshift makes the inverse transform
+    ne - 1 points long and the crop [p, ne - p) keeps min(n, ne - 1 - p) of
+    them (src/bldfm/solver.py:265-290).  The oracle's index maps equal that
+    literal sequence on its own spectra, and a dispersion solve raises the
+    reference's IndexError (the truncated source spectrum has nl + 1 modes)."""
+    from numpy.fft import fftshift, ifftshift
+
+    z, profs = opbl.vertical_profiles(10, 5.0, (3.0, 1.0), 0.3, mol=-30.0)
+    domain = (float(nx), float(ny))
+    kw = dict(modes=modes, meas_pt=(10.0, 7.0), footprint=True, precision="double", halo=halo)
+    (_, conc, flx), (tp, tq) = osolver.steady_state_transport_solver(np.zeros((ny, nx)), z, profs, domain, 6,
+                                                                     _return_spectra=True, **kw)
+    px, py = int(halo / (domain[0] / nx)), int(halo / (domain[1] / ny))
+    nxe, nye = nx + 2 * px, ny + 2 * py
+    dlx, dly = (nxe - modes[0]) // 2, (nye - modes[1]) // 2
+    for spec, out in ((tp, conc), (tq, flx)):
+        full = ifftshift(np.pad(fftshift(spec, axes=(1, 2)), ((0, 0), (dly, dly), (dlx, dlx))), axes=(1, 2))
+        ref = np.squeeze(np.fft.fft2(full, norm="backward").real[:, py : nye - py, px : nxe - px])
+        assert ref.shape == out.shape and np.array_equal(ref, out)
+    with pytest.raises(IndexError, match="boolean index did not match"):
+        osolver.steady_state_transport_solver(np.ones((ny, nx)), z, profs, domain, 6, modes=modes, halo=halo)
