@@ -46,12 +46,20 @@
 #define SUR_NPARAM_PAD ((SUR_NPARAM + 255) / 256 * 256)
 
 // Fixed (trained-with) constants of a network, set once by the host.
+//
+// mode 0 is the design above.  mode 1 is the network exactly as the
+// surrogate's specification words it: x0 = [RFF(kx, ky) (64) | 6 met scalars
+// (u*, 1/L, ln z0, zm, |U|, direction; normalised) | 0], no prior, and the two
+// outputs are Re G and Im G themselves (rff[j][2..3] unused).
+#define SUR_MODE_SPEC 1
+#define SUR_NMET_SPEC 6
 struct SurStatic {
   float rff[SUR_NRFF][SUR_NC];  // B
   float cscale[SUR_NC];         // coordinate scales
   float met_mu[SUR_NMET];       // met normalisation: (m - mu) / sd
   float met_sd[SUR_NMET];
-  float pad[64 - 2 * SUR_NMET - SUR_NC];
+  float mode;                   // 0 or SUR_MODE_SPEC
+  float pad[64 - 2 * SUR_NMET - SUR_NC - 1];
 };
 
 // Parameters in the layout the kernels read, written by the packing kernel.

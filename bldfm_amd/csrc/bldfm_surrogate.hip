@@ -81,6 +81,28 @@ __device__ __forceinline__ float2 wkb_phi0(cmet_t& M, float lx, float ly) {
 __device__ __forceinline__ void sur_features(cstat_t& S, cmet_t& M, float lx, float ly, int part,
                                              __bf16* act, float2* p0s, int pt, __bf16* __restrict__ xs,
                                              int64_t N, int64_t gp) {
+  auto put = [&](int f, float v) {
+    const __bf16 b = (__bf16)v;
+    act[sur_act_off(pt, f)] = b;
+    if (xs) xs[(int64_t)f * N + gp] = b;
+  };
+  const int j0 = part * (SUR_NRFF / 2);
+  if (S.mode == (float)SUR_MODE_SPEC) {  // RFF of (kx, ky), 6 met scalars, no prior
+    const float c0 = lx * S.cscale[0], c1 = ly * S.cscale[1];
+    for (int j = j0; j < j0 + SUR_NRFF / 2; ++j) {
+      float rev = S.rff[j][0] * c0 + S.rff[j][1] * c1;
+      rev -= rintf(rev);
+      put(j, __builtin_amdgcn_sinf(rev));
+      put(SUR_NRFF + j, __builtin_amdgcn_cosf(rev));
+    }
+    if (part == 1) {
+      for (int d = 0; d < SUR_NMET_SPEC; ++d) put(2 * SUR_NRFF + d, (M.met[d] - S.met_mu[d]) / S.met_sd[d]);
+      for (int f = 2 * SUR_NRFF + SUR_NMET_SPEC; f < SUR_IN; ++f) put(f, 0.0f);
+    } else {
+      p0s[pt] = make_float2(0.0f, 0.0f);
+    }
+    return;
+  }
   const float2 phi0 = wkb_phi0(M, lx, ly);
   const float kpar = (lx * M.wx + ly * M.wy) * M.zm;
   const float kperp = fabsf(ly * M.wx - lx * M.wy) * M.zm;
@@ -89,12 +111,6 @@ __device__ __forceinline__ void sur_features(cstat_t& S, cmet_t& M, float lx, fl
   c[1] = kperp * S.cscale[1];
   c[2] = copysignf(sqrtf(fabsf(kpar)), kpar) * S.cscale[2];
   c[3] = sqrtf(kperp) * S.cscale[3];
-  auto put = [&](int f, float v) {
-    const __bf16 b = (__bf16)v;
-    act[sur_act_off(pt, f)] = b;
-    if (xs) xs[(int64_t)f * N + gp] = b;
-  };
-  const int j0 = part * (SUR_NRFF / 2);
   for (int j = j0; j < j0 + SUR_NRFF / 2; ++j) {
     // phase in revolutions, reduced exactly to [-1/2, 1/2]; the hardware
     // sine and cosine take revolutions
@@ -312,7 +328,14 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_fwd(const uint8_t* __res
   __syncthreads();
   float li = 0.f;
   if (tid < SUR_PTS && !valid) dphi[p0 + tid] = make_float2(0.f, 0.f);
-  if (tid < SUR_PTS && valid) {
+  if (tid < SUR_PTS && valid && S.mode == (float)SUR_MODE_SPEC) {  // outputs are (Re G, Im G)
+    const float2 P = phis[tid];
+    const float er = P.x - gr, ei = P.y - gi;
+    li = er * er + ei * ei;
+    const float sc = 1.0f / ((Sb + 1e-30f) * (float)gridDim.x);
+    li *= sc;
+    dphi[p0 + tid] = make_float2(2.0f * er * sc, 2.0f * ei * sc);
+  } else if (tid < SUR_PTS && valid) {
     const float2 P = phis[tid];
     const float mag = __expf(-fmaxf(P.x, -20.0f));
     float sn, cs;
@@ -572,7 +595,8 @@ __global__ __launch_bounds__(256) void k_sur_adam(float* __restrict__ p, float* 
 
 // ---------------------------------------------------------------- inference in a plan
 // Footprint spectrum of accepted surrogate instances: per lane
-// G = exp(-Phi) with Re Phi >= 0 (|G| <= 1), X_q = q00 G, X_p = 0, packed and
+// G = exp(-Phi) with Re Phi >= 0 (|G| <= 1; in mode SUR_MODE_SPEC the two outputs
+// are Re G, Im G), X_q = q00 G, X_p = 0, packed and
 // phase-shifted like the shooting kernel's output; the (0,0) mode is exact (G = 1).
 __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_plan(PlanDev P, ExecBufs B) {
   __shared__ __attribute__((aligned(16))) __bf16 act[SUR_PTS * SUR_W];
@@ -602,10 +626,16 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_plan(PlanDev P, ExecBufs
       emit_origin(D, B.ws, 0, 0.0, D.q00);
     } else {
       const float2 ph = phis[tid];
-      const double mag = exp(-fmax((double)ph.x, 0.0)) * D.q00;
-      double sn, cs;
-      sincos((double)ph.y, &sn, &cs);
-      emit_mode(D, L, B.ws, 0, cmk(0.0, 0.0), cmk(mag * cs, -mag * sn), phase_shift(D, L));
+      double2 g;
+      if (S.mode == (float)SUR_MODE_SPEC) {
+        g = cmk((double)ph.x, (double)ph.y);
+      } else {
+        const double mag = exp(-fmax((double)ph.x, 0.0));
+        double sn, cs;
+        sincos((double)ph.y, &sn, &cs);
+        g = cmk(mag * cs, -mag * sn);
+      }
+      emit_mode(D, L, B.ws, 0, cmk(0.0, 0.0), cmk(g.x * D.q00, g.y * D.q00), phase_shift(D, L));
     }
   }
 }

@@ -216,10 +216,19 @@ class TrainLog:
     loss: list = field(default_factory=list)
 
 
-class SurrogateNet:
-    """Parameters, Adam state and packed buffer of one network."""
+MODE_SPEC = 1  # the network as specified: RFF(kx, ky) + 6 met scalars -> (Re G, Im G), no prior
+STATIC_MODE = 148  # index of the mode in the feature header
 
-    def __init__(self, met_mu, met_sd, hull: Hull = Hull(), seed=0, rff_sigma=2.0, dev=None):
+
+class SurrogateNet:
+    """Parameters, Adam state and packed buffer of one network.
+
+    ``mode`` 0 (default): wind-frame, zm-scaled coordinates, 8 met inputs and
+    the WKB prior; the outputs correct Phi = -log G.  ``mode`` MODE_SPEC: 64
+    random Fourier features of the raw wavenumber (kx, ky), 6 normalised met
+    scalars (u*, 1/L, ln z0, zm, |U|, direction), outputs Re G and Im G."""
+
+    def __init__(self, met_mu, met_sd, hull: Hull = Hull(), seed=0, rff_sigma=2.0, dev=None, mode=0):
         self.device = dev if dev is not None else _lib.device()
         self.hull = hull
         nparam, net_bytes, bpp, blk = sizes()
@@ -227,10 +236,16 @@ class SurrogateNet:
         rng = np.random.default_rng(seed)
         static = np.zeros(STATIC_FLOATS, dtype=np.float32)
         static[: N_RFF * N_COORD] = rng.normal(0.0, rff_sigma, N_RFF * N_COORD)
-        kz = hull.kmax * hull.zm[1] * math.sqrt(2.0)  # largest |k| zm
-        static[128:132] = [1.0 / kz, 1.0 / kz, 1.0 / math.sqrt(kz), 1.0 / math.sqrt(kz)]
+        if mode == MODE_SPEC:  # B acts on (kx, ky) scaled by 1/kmax
+            static[: N_RFF * N_COORD].reshape(N_RFF, N_COORD)[:, 2:] = 0.0
+            static[128:132] = [1.0 / hull.kmax, 1.0 / hull.kmax, 0.0, 0.0]
+        else:
+            kz = hull.kmax * hull.zm[1] * math.sqrt(2.0)  # largest |k| zm
+            static[128:132] = [1.0 / kz, 1.0 / kz, 1.0 / math.sqrt(kz), 1.0 / math.sqrt(kz)]
         static[132:140] = np.asarray(met_mu, dtype=np.float32)
         static[140:148] = np.asarray(met_sd, dtype=np.float32)
+        static[STATIC_MODE] = float(mode)
+        self.mode = int(mode)
         self.static = torch.from_numpy(static)
         self.params = torch.from_numpy(self._init_params(rng)).to(self.device)
         self.m = torch.zeros_like(self.params)
@@ -274,10 +289,12 @@ class SurrogateNet:
                                                  _lib.stream_handle(self.device)), "sur_forward")
         return out
 
-    @staticmethod
-    def transfer(phi):
-        """G = exp(-Phi) with |G| <= 1, complex128."""
+    def transfer(self, phi):
+        """G from the network outputs, complex128: exp(-Phi) with |G| <= 1, or
+        the outputs themselves in MODE_SPEC."""
         p = phi.double()
+        if self.mode == MODE_SPEC:
+            return torch.complex(p[:, 0], p[:, 1])
         return torch.exp(-torch.complex(p[:, 0].clamp(min=0.0), p[:, 1]))
 
     def grad_step(self, metset: MetSet, blk_met, kxy, g, ws=None, loss=None, phi=None):
@@ -386,7 +403,7 @@ class SurrogateNet:
         hull = Hull(ustar=(h[0], h[1]), invl=(h[2], h[3]), z0=(h[4], h[5]), z0_zm_max=h[6], zm=(h[7], h[8]),
                     kmax=h[9], top_ratio=h[10], n=int(h[11]))
         static = st["static"].numpy()
-        net = cls(static[132:140], static[140:148], hull=hull, dev=dev)
+        net = cls(static[132:140], static[140:148], hull=hull, dev=dev, mode=int(static[STATIC_MODE]))
         net.static = st["static"].clone()
         net.net[: 4 * STATIC_FLOATS].copy_(net.static.view(torch.uint8).to(net.device))
         net.params.copy_(st["params"].to(net.device))
@@ -433,11 +450,85 @@ def train_default(steps=400000, pool=22000, seed=1, heldout=48, out=None, log=pr
     return net, prior, err
 
 
+SPEC_CHECKPOINT = DEFAULT_CHECKPOINT.parent / "surrogate_spec_c4.pt"
+
+
+def spec_dataset(metset: MetSet, domain=(1000.0, 1000.0), modes=(256, 256), chunk=64):
+    """G of every sample on one fixed mode lattice (the 256^2 modes of a 256^2
+    footprint on ``domain``: with 1000 m the production wavenumber range at
+    half the density), solved in FP64, kept as fp32 pairs (n, P, 2)."""
+    lat = torch.from_numpy(lattice(domain, modes)).to(metset.device)
+    P = lat.shape[0]
+    assert P % TRANSFER_BLOCK == 0
+    data = torch.empty((metset.n, P, 2), dtype=torch.float32, device=metset.device)
+    for c0 in range(0, metset.n, chunk):
+        c1 = min(c0 + chunk, metset.n)
+        blk = torch.arange(c0, c1, dtype=torch.int32, device=metset.device).repeat_interleave(P // TRANSFER_BLOCK)
+        data[c0:c1] = metset.solve(blk, lat.repeat(c1 - c0, 1)).view(c1 - c0, P, 2).float()
+    return lat, data
+
+
+def train_spec(samples=20000, steps=200000, batch=65536, lr=1e-3, warmup=200, seed=5, heldout=32, out=None,
+               log=print):
+    """The network exactly as specified: ``samples`` seeded met draws (those
+    the FP64 solver resolves), their G on a fixed 256^2 lattice, 64 RFF of
+    (kx, ky) + 6 met scalars -> (Re G, Im G), batches of ``batch`` lattice
+    points (half uniform, half from the quarter of the lattice nearest k = 0),
+    Adam lr 1e-3 with cosine decay.  Returns the network, its held-out
+    footprint rel-L2 on the production 512^2 lattice and the logged losses."""
+    import time
+
+    ms = MetSet(sample_met(samples, seed=seed))
+    ms = ms.subset(ms.conditioned(ms.hull.kmax))
+    ho = MetSet(sample_met(heldout, seed=seed + 1))
+    ho = ho.subset(ho.conditioned(ho.hull.kmax))
+    lat, data = spec_dataset(ms)
+    P = lat.shape[0]
+    near = torch.argsort(lat.norm(dim=1))[: P // 4]  # lattice points nearest k = 0
+    mu, sd = met_normalisation(ms)
+    net = SurrogateNet(mu, sd, seed=0, mode=MODE_SPEC)
+    log(f"spec network: {ms.n} samples x {P} lattice points ({data.numel() * 4 / 2**30:.1f} GiB), held out {ho.n}")
+    nblk = batch // net.block
+    gen = torch.Generator(device=net.device)
+    gen.manual_seed(seed)
+    ws = torch.empty(batch * net.bytes_per_point, dtype=torch.uint8, device=net.device)
+    loss = torch.zeros(1, dtype=torch.float32, device=net.device)
+    torch.cuda.synchronize(net.device)
+    t0 = time.perf_counter()
+    losses = []
+    for it in range(steps):
+        sid = torch.randint(0, ms.n, (nblk,), generator=gen, device=net.device)
+        pid = torch.randint(0, P, (nblk, net.block), generator=gen, device=net.device)
+        pid[:, net.block // 2 :] = near[torch.randint(0, near.numel(), (nblk, net.block // 2), generator=gen,
+                                                      device=net.device)]
+        g = data[sid[:, None], pid].reshape(-1, 2).double()
+        kxy = lat[pid.reshape(-1)]
+        loss.zero_()
+        net.grad_step(ms, sid.to(torch.int32), kxy, g, ws=ws, loss=loss)
+        scale = min(1.0, (it + 1) / warmup) * 0.5 * (1.0 + math.cos(math.pi * it / steps))
+        net.adam(lr * scale)
+        if it % max(steps // 20, 1) == 0 or it == steps - 1:
+            losses.append((it, float(loss.item())))
+            log(f"  step {it} loss {losses[-1][1]:.4e}")
+    torch.cuda.synchronize(net.device)
+    err = net.evaluate(ho)
+    log(f"{steps} steps in {time.perf_counter() - t0:.1f} s; held-out footprint rel-L2 median "
+        f"{np.median(err):.3e} p90 {np.percentile(err, 90):.3e} max {err.max():.3e}")
+    if out is not None:
+        net.save(out, with_optimizer=False)
+    return net, err, losses
+
+
 if __name__ == "__main__":
     import argparse
 
     ap = argparse.ArgumentParser(description="Train the footprint-transfer surrogate on the GPU")
-    ap.add_argument("--steps", type=int, default=400000)
-    ap.add_argument("--out", default=str(DEFAULT_CHECKPOINT))
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--spec", action="store_true",
+                    help="the network exactly as specified (RFF of (kx, ky) + 6 met -> Re, Im; 20,000 samples)")
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
-    train_default(a.steps, out=a.out, log=lambda m: print(m, flush=True))
+    if a.spec:
+        train_spec(steps=a.steps or 200000, out=a.out or str(SPEC_CHECKPOINT), log=lambda m: print(m, flush=True))
+    else:
+        train_default(a.steps or 400000, out=a.out or str(DEFAULT_CHECKPOINT), log=lambda m: print(m, flush=True))
