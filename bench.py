@@ -18,7 +18,7 @@ one footprint per host core at a time, like the reference's process pool.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                     [--surrogate CHECKPOINT] [--tol 1e-3] [--no-surrogate] [--no-tune]
-                    [--cpu-sample 16] [--cpu-workers 16]
+                    [--cpu-sample 16] [--cpu-workers 16] [--host-steps 10] [--interface-steps 128]
     python bench.py --copy-calibration   # 1 GiB device copy, the PMC byte reference
 
 The surrogate leg serves the same batch with the trained network
@@ -27,6 +27,11 @@ an error gate and solving a footprint in FP64 when that check exceeds
 ``--tol``; it reports its footprints/s, the accepted/fallback split, the
 footprint and footprint-integral errors against the FP64 batch and the
 network kernel's bf16 MFMA throughput.
+
+Two host-inclusive legs report what a caller that needs the footprints in
+host memory gets: the same batches with every footprint copied to pinned host
+memory (copies overlapped with the next batch), and the reference's
+``run_bldfm_timeseries`` interface end to end.  Neither is ``value``.
 """
 
 import argparse
@@ -76,17 +81,41 @@ def workload(batch, seed=20261204):
     return specs
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "footprint_batch_512x64_pmc.txt")
+
+
+def kernel_source_hash():
+    """sha256 of the HIP library's sources (bldfm_amd/csrc and the C header),
+    which identifies the kernels a profile was taken on."""
+    import hashlib
+
+    h = hashlib.sha256()
+    files = sorted(os.listdir(os.path.join(ROOT, "bldfm_amd", "csrc")))
+    paths = [os.path.join(ROOT, "bldfm_amd", "csrc", f) for f in files if f.endswith((".hip", ".h"))]
+    for path in paths + [os.path.join(ROOT, "include", "bldfm_amd.h")]:
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def measured_traffic(kernel="k_shoot"):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc
-    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", "footprint_batch_512x64_pmc.txt")
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), and
+    whether that summary was taken on the current kernel sources (its
+    ``source_sha256`` line).  A summary of other sources gives no figure."""
+    value, sha = None, None
     try:
-        for line in open(path):
+        for line in open(PMC_SUMMARY):
+            if line.startswith("source_sha256:"):
+                sha = line.split(":")[1].strip()
             if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-                return float(line.split(":")[1])
+                value = float(line.split(":")[1])
     except OSError:
-        pass
-    return None
+        return None, "no PMC summary"
+    if sha != kernel_source_hash():
+        return None, "the committed PMC summary was taken on other kernel sources"
+    return value, "rocprofv3 FETCH_SIZE+WRITE_SIZE of these kernel sources, copy-calibrated (profiles/)"
 
 
 def timed(batch_obj, steps, warmup):
@@ -237,6 +266,88 @@ def fft_roofline(fb, phases, n):
     return out
 
 
+def host_inclusive_leg(fb, steps):
+    """The FP64 batch with every footprint's conc and flx copied to pinned
+    host memory: two output buffers, each batch's device-to-host copy on a
+    second stream overlapping the next batch's kernels.  Footprints per
+    second of wall time, host copies included."""
+    import torch
+
+    comp = torch.cuda.current_stream()
+    copy = torch.cuda.Stream()
+    outs = [(fb.conc, fb.flx), (torch.empty_like(fb.conc), torch.empty_like(fb.flx))]
+    host = [tuple(torch.empty(fb.conc.numel(), dtype=fb.conc.dtype, pin_memory=True) for _ in range(2))
+            for _ in range(2)]
+    done_comp = [torch.cuda.Event() for _ in range(2)]
+    done_copy = [torch.cuda.Event() for _ in range(2)]
+
+    def step(k):
+        b = k % 2
+        comp.wait_event(done_copy[b])  # the copy that last read these buffers has finished
+        fb.plan.execute(fb.prof, None, outs[b][0], outs[b][1], fb.workspace)
+        done_comp[b].record(comp)
+        copy.wait_event(done_comp[b])
+        with torch.cuda.stream(copy):
+            host[b][0].copy_(outs[b][0], non_blocking=True)
+            host[b][1].copy_(outs[b][1], non_blocking=True)
+        done_copy[b].record(copy)
+
+    for b in range(2):
+        done_copy[b].record(copy)
+    for k in range(2):
+        step(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(k)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # device-to-host bandwidth alone, on the same buffers
+    t1 = time.perf_counter()
+    for k in range(steps):
+        host[k % 2][0].copy_(outs[k % 2][0], non_blocking=True)
+        host[k % 2][1].copy_(outs[k % 2][1], non_blocking=True)
+    torch.cuda.synchronize()
+    d2h = 2 * fb.conc.numel() * fb.conc.element_size()
+    ok = all(torch.equal(host[b][i], outs[b][i].cpu()) for b in range(2) for i in range(2))
+    del outs, host
+    return {"value": round(len(fb) * steps / dt, 2), "unit": "footprints/s",
+            "bytes_to_host_per_step": d2h, "d2h_GBps": round(d2h * steps / (time.perf_counter() - t1) / 1e9, 2),
+            "copies_verified": bool(ok),
+            "how": "two output buffers; each batch's conc+flx copied to pinned host memory on a second stream "
+                   "while the next batch runs"}
+
+
+def interface_leg(specs, nsteps):
+    """``run_bldfm_timeseries`` (the reference's interface) over ``nsteps``
+    hourly steps of the same configuration: plan set-up, solves, copies to
+    the host and the reference's result dictionaries (grid arrays per result)
+    included.  Footprints per second of wall time."""
+    from bldfm_amd.config_parser import BLDFMConfig, DomainConfig, MetConfig, SolverConfig, TowerConfig
+    from bldfm_amd.interface import run_bldfm_timeseries
+
+    s = specs[0]
+    rng = np.random.default_rng(7)
+    cfg = BLDFMConfig(
+        domain=DomainConfig(nx=s.nx, ny=s.ny, xmax=s.domain[0], ymax=s.domain[1], nz=s.n, modes=s.modes),
+        towers=[TowerConfig(name="t", lat=0.0, lon=0.0, z_m=s.meas_height, x=s.meas_pt[0], y=s.meas_pt[1])],
+        met=MetConfig(ustar=[float(v) for v in rng.uniform(0.2, 0.5, nsteps)],
+                      mol=[float(v) for v in rng.choice([-20.0, 100.0], nsteps)],
+                      wind_speed=[float(v) for v in rng.uniform(2.0, 6.0, nsteps)],
+                      wind_dir=[float(v) for v in rng.uniform(0.0, 360.0, nsteps)]),
+        solver=SolverConfig(closure="MOST", precision="double", footprint=True),
+    )
+    run_bldfm_timeseries(cfg, cfg.towers[0])  # plan caches and device warm-up
+    t0 = time.perf_counter()
+    res = run_bldfm_timeseries(cfg, cfg.towers[0])
+    dt = time.perf_counter() - t0
+    return {"value": round(nsteps / dt, 2), "unit": "footprints/s", "timesteps": nsteps,
+            "output_shape": list(np.shape(res[0]["flx"])),
+            "how": "run_bldfm_timeseries on the same grid, modes and tower (the interface's own column: "
+                   "nz points to 2 zm, output at zm): met -> device profiles -> batched solve -> host copies "
+                   "-> result dicts with their grid arrays"}
+
+
 def copy_calibration():
     """A plain device copy of 1 GiB (three times): profiled with rocprofv3
     --pmc FETCH_SIZE / WRITE_SIZE, it gives the factors that turn those
@@ -267,6 +378,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuned one")
+    ap.add_argument("--host-steps", type=int, default=10,
+                    help="batches of the leg that copies every footprint to host memory (0: skip it)")
+    ap.add_argument("--interface-steps", type=int, default=128,
+                    help="timesteps of the run_bldfm_timeseries leg (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -333,11 +448,20 @@ def main():
                   f"FP64 solver), {sl['accepted']} accepted / {sl['fallback']} fallback", file=sys.stderr, flush=True)
         extra["surrogate"] = sl
 
+    if args.host_steps > 0:
+        extra["host_inclusive"] = host_inclusive_leg(fb, args.host_steps)
+        print(f"[bench] host-inclusive: {extra['host_inclusive']['value']:.1f} footprints/s "
+              f"(device-to-host {extra['host_inclusive']['d2h_GBps']:.1f} GB/s)", file=sys.stderr, flush=True)
+    if args.interface_steps > 0:
+        extra["interface"] = interface_leg(specs, args.interface_steps)
+        print(f"[bench] interface: {extra['interface']['value']:.1f} footprints/s", file=sys.stderr, flush=True)
+
     cpu_fps, worst, cpu_s = (cpu_baseline(specs, fb, args.cpu_sample, workers, pool_ctx) if args.cpu_sample > 0
                              else (None, None, None))
     if cpu_fps is not None:
         print(f"[bench] cpu baseline: {cpu_fps:.3f} footprints/s on {workers} cores "
               f"({args.cpu_sample} footprints, {cpu_s:.1f} s of CPU time)", file=sys.stderr, flush=True)
+    traffic, traffic_src = measured_traffic() if args.batch == 128 else (None, "PMC summary is of batch 128")
     out = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -364,8 +488,10 @@ def main():
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64, 4),
-            "traffic": measured_traffic() if args.batch == 128 else None,
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated; profiles/)",
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch",
+            "traffic_source": traffic_src,
+            "kernel_source_sha256": kernel_source_hash(),
             "algorithmic_bytes": 16 * sum(2 * (l - 1) for l, _ in fb.lanes_and_steps()),
             "flop_per_unit": FLOP_PER_MODE_STEP,
             "units_per_launch": units,

@@ -135,16 +135,25 @@ __device__ __forceinline__ void sur_features(cstat_t& S, cmet_t& M, float lx, fl
 // KB16 steps of 16, W in forward fragment order (sur_pack_fwd / sur_pack_bwd).
 // Element j of acc[a][pb] in lane (half, r) is output feature
 // 64 wave + 32 a + 8 (j/4) + 4 half + j%4 of point 32 pb + r.
+// With bias, the accumulators start from it (the epilogue then adds nothing).
 template <int KB16, int UNROLL>
 __device__ __forceinline__ void mfma_tile(const bf16x8* __restrict__ Wp, const __bf16* act, int wave,
-                                          int lane, f32x16 (&acc)[2][4]) {
+                                          int lane, f32x16 (&acc)[2][4], const float* bias = nullptr) {
+  const int r = lane & 31, half = lane >> 5;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int jq = 0; jq < 4; ++jq) {
+      float4 b4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (bias) b4 = *(const float4*)&bias[64 * wave + 32 * a + 8 * jq + 4 * half];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) acc[a][b][j] = 0.0f;
-  const int r = lane & 31, half = lane >> 5;
+      for (int b = 0; b < 4; ++b) {
+        acc[a][b][4 * jq + 0] = b4.x;
+        acc[a][b][4 * jq + 1] = b4.y;
+        acc[a][b][4 * jq + 2] = b4.z;
+        acc[a][b][4 * jq + 3] = b4.w;
+      }
+    }
   const bf16x8* A0p = Wp + (int64_t)(wave * 2 + 0) * KB16 * 64 + lane;
   const bf16x8* A1p = Wp + (int64_t)(wave * 2 + 1) * KB16 * 64 + lane;
   // weight fragments come from L2: unrolled, the scheduler issues later
@@ -174,10 +183,6 @@ __device__ __forceinline__ void lds_put4(__bf16* act, int pt, int o0, const floa
   *(bf16x4*)(act + sur_act_off(pt, o0)) = w;
 }
 
-// Hidden layers and output of the points whose inputs are in `act`, with
-// biases and output weights from `dyn` (LDS).  Leaves Phi = Phi0 + network
-// output of local point t (t < SUR_PTS) in phis[t].  TRAIN: stores z_l
-// and x_{l+1} (bf16) feature-major at point offset p0.
 // Copy of the network's biases and output layer into LDS (dyn), read by the
 // epilogues instead of L2.
 __device__ __forceinline__ void stage_dyn(const uint8_t* __restrict__ net, SurDyn* dsh) {
@@ -185,76 +190,105 @@ __device__ __forceinline__ void stage_dyn(const uint8_t* __restrict__ net, SurDy
   for (int i = threadIdx.x; i < (int)(sizeof(SurDyn) / 16); i += blockDim.x) ((float4*)dsh)[i] = src[i];
 }
 
-template <bool TRAIN>
+// Epilogue of hidden layer l for one wave's accumulators: x = SiLU(z) into the
+// LDS tile (LAST: folded into the two fp32 outputs, per-wave partial sums in
+// red).  FOLD: z is the accumulator itself (the products started from the
+// biases).  TRAIN: also stores z_l and x_{l+1} (bf16) feature-major at point
+// offset p0.
+template <bool TRAIN, bool FOLD, bool LAST>
+__device__ __forceinline__ void layer_epilogue(const f32x16 (&acc)[2][4], const SurDyn* dyn, int l, __bf16* act,
+                                               float* red, __bf16* __restrict__ xs, __bf16* __restrict__ zs,
+                                               int64_t N, int64_t p0, int wave, int lane) {
+  const int r = lane & 31, half = lane >> 5;
+  float po[4][2];
+#pragma unroll
+  for (int pb = 0; pb < 4; ++pb) po[pb][0] = po[pb][1] = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+    for (int jq = 0; jq < 4; ++jq) {
+      const int o0 = 64 * wave + 32 * a + 8 * jq + 4 * half;
+      float bb[4] = {0.f, 0.f, 0.f, 0.f};
+      if (!FOLD) {
+        const float4 b4 = *(const float4*)&dyn->b[l][o0];
+        bb[0] = b4.x, bb[1] = b4.y, bb[2] = b4.z, bb[3] = b4.w;
+      }
+      float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0;
+      if (LAST) {
+        w0 = *(const float4*)&dyn->wo[0][o0];
+        w1 = *(const float4*)&dyn->wo[1][o0];
+      }
+#pragma unroll
+      for (int pb = 0; pb < 4; ++pb) {
+        const int pt = 32 * pb + r;
+        float z[4], x[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) z[e] = FOLD ? acc[a][pb][4 * jq + e] : acc[a][pb][4 * jq + e] + bb[e];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = silu_f(z[e]);
+        if (TRAIN) {
+          __bf16* zr = zs + ((int64_t)l * SUR_W + o0) * N + p0 + pt;
+          __bf16* xr = xs + xs_off(l + 1, N) + (int64_t)o0 * N + p0 + pt;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            zr[e * N] = (__bf16)z[e];
+            xr[e * N] = (__bf16)x[e];
+          }
+        }
+        if (!LAST) {
+          lds_put4(act, pt, o0, x);
+        } else {
+          po[pb][0] += w0.x * x[0] + w0.y * x[1] + w0.z * x[2] + w0.w * x[3];
+          po[pb][1] += w1.x * x[0] + w1.y * x[1] + w1.z * x[2] + w1.w * x[3];
+        }
+      }
+    }
+  }
+  if (LAST) {
+#pragma unroll
+    for (int pb = 0; pb < 4; ++pb) {
+      const float v0 = po[pb][0] + __shfl_xor(po[pb][0], 32);
+      const float v1 = po[pb][1] + __shfl_xor(po[pb][1], 32);
+      if (half == 0) {
+        red[(wave * SUR_PTS + 32 * pb + r) * 2 + 0] = v0;
+        red[(wave * SUR_PTS + 32 * pb + r) * 2 + 1] = v1;
+      }
+    }
+  }
+}
+
+// Hidden layers and output of the points whose inputs are in `act`, with
+// biases and output weights from `dyn` (LDS).  Leaves Phi = Phi0 + network
+// output of local point t (t < SUR_PTS) in phis[t].  TRAIN: stores z_l
+// and x_{l+1} (bf16) feature-major at point offset p0.  FOLD (serving): the
+// products start from the biases, so the epilogue is SiLU and the store alone.
+template <bool TRAIN, bool FOLD = false>
 __device__ __forceinline__ void sur_hidden(const uint8_t* __restrict__ net, const SurDyn* dyn, __bf16* act,
                                            float* red, const float2* p0s, float2* phis,
                                            __bf16* __restrict__ xs, __bf16* __restrict__ zs, int64_t N,
                                            int64_t p0) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 31, half = lane >> 5;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  constexpr int U = TRAIN ? 1 : SUR_W / 32;  // the training kernel has no registers to spare
   f32x16 acc[2][4];
-  for (int l = 0; l < SUR_NL; ++l) {
-    constexpr int U = TRAIN ? 1 : SUR_W / 16;  // the training kernel has no registers to spare
-    if (l == 0)
-      mfma_tile<SUR_IN / 16, TRAIN ? 1 : SUR_IN / 16>((const bf16x8*)(net + SUR_OFF_W0P), act, wave, lane, acc);
-    else
-      mfma_tile<SUR_W / 16, U>((const bf16x8*)(net + SUR_OFF_WP(l)), act, wave, lane, acc);
-    __syncthreads();  // every wave has read this layer's input tile
-    const bool last = (l == SUR_NL - 1);
-    float po[4][2];
-#pragma unroll
-    for (int pb = 0; pb < 4; ++pb) po[pb][0] = po[pb][1] = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-#pragma unroll
-      for (int jq = 0; jq < 4; ++jq) {
-        const int o0 = 64 * wave + 32 * a + 8 * jq + 4 * half;
-        const float4 bb = *(const float4*)&dyn->b[l][o0];
-        float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0;
-        if (last) {
-          w0 = *(const float4*)&dyn->wo[0][o0];
-          w1 = *(const float4*)&dyn->wo[1][o0];
-        }
-#pragma unroll
-        for (int pb = 0; pb < 4; ++pb) {
-          const int pt = 32 * pb + r;
-          float z[4], x[4];
-          z[0] = acc[a][pb][4 * jq + 0] + bb.x;
-          z[1] = acc[a][pb][4 * jq + 1] + bb.y;
-          z[2] = acc[a][pb][4 * jq + 2] + bb.z;
-          z[3] = acc[a][pb][4 * jq + 3] + bb.w;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) x[e] = silu_f(z[e]);
-          if (TRAIN) {
-            __bf16* zr = zs + ((int64_t)l * SUR_W + o0) * N + p0 + pt;
-            __bf16* xr = xs + xs_off(l + 1, N) + (int64_t)o0 * N + p0 + pt;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              zr[e * N] = (__bf16)z[e];
-              xr[e * N] = (__bf16)x[e];
-            }
-          }
-          if (!last) {
-            lds_put4(act, pt, o0, x);
-          } else {
-            po[pb][0] += w0.x * x[0] + w0.y * x[1] + w0.z * x[2] + w0.w * x[3];
-            po[pb][1] += w1.x * x[0] + w1.y * x[1] + w1.z * x[2] + w1.w * x[3];
-          }
-        }
-      }
-    }
-    if (last) {
-#pragma unroll
-      for (int pb = 0; pb < 4; ++pb) {
-        const float v0 = po[pb][0] + __shfl_xor(po[pb][0], 32);
-        const float v1 = po[pb][1] + __shfl_xor(po[pb][1], 32);
-        if (half == 0) {
-          red[(wave * SUR_PTS + 32 * pb + r) * 2 + 0] = v0;
-          red[(wave * SUR_PTS + 32 * pb + r) * 2 + 1] = v1;
-        }
-      }
-    }
+  // layer 0, the middle layers and the last layer as straight-line code: with
+  // the epilogue kinds in branches of one loop body the compiler hoists their
+  // common arithmetic above the branch, and the hoisted values spill
+  mfma_tile<SUR_IN / 16, TRAIN ? 1 : SUR_IN / 16>((const bf16x8*)(net + SUR_OFF_W0P), act, wave, lane, acc,
+                                                  FOLD ? dyn->b[0] : nullptr);
+  __syncthreads();  // every wave has read this layer's input tile
+  layer_epilogue<TRAIN, FOLD, false>(acc, dyn, 0, act, red, xs, zs, N, p0, wave, lane);
+  __syncthreads();
+  for (int l = 1; l < SUR_NL - 1; ++l) {
+    mfma_tile<SUR_W / 16, U>((const bf16x8*)(net + SUR_OFF_WP(l)), act, wave, lane, acc, FOLD ? dyn->b[l] : nullptr);
+    __syncthreads();
+    layer_epilogue<TRAIN, FOLD, false>(acc, dyn, l, act, red, xs, zs, N, p0, wave, lane);
     __syncthreads();
   }
+  mfma_tile<SUR_W / 16, U>((const bf16x8*)(net + SUR_OFF_WP(SUR_NL - 1)), act, wave, lane, acc,
+                           FOLD ? dyn->b[SUR_NL - 1] : nullptr);
+  __syncthreads();
+  layer_epilogue<TRAIN, FOLD, true>(acc, dyn, SUR_NL - 1, act, red, xs, zs, N, p0, wave, lane);
+  __syncthreads();
   if (tid < SUR_PTS) {
     float s0 = dyn->bo[0] + p0s[tid].x, s1 = dyn->bo[1] + p0s[tid].y;
 #pragma unroll
@@ -619,7 +653,7 @@ __global__ __launch_bounds__(SUR_THREADS, 2) void k_sur_plan(PlanDev P, ExecBufs
   }
   stage_dyn(P.sur_net, &dsh);
   __syncthreads();
-  sur_hidden<false>(P.sur_net, &dsh, act, red, p0s, phis, nullptr, nullptr, 0, 0);
+  sur_hidden<false, true>(P.sur_net, &dsh, act, red, p0s, phis, nullptr, nullptr, 0, 0);
   Lane L;
   if (tid < SUR_PTS && lane_setup(D, tr.local * SUR_PTS + pt, L)) {
     if (L.kind == LANE_ORIGIN) {
