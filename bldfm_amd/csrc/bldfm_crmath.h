@@ -334,6 +334,10 @@ BLDFM_HD void gl_exp_table(int k, double* H, double* T) {
 // glibc's exp (sysdeps/ieee754/dbl-64/e_exp.c, the ARM optimized-routines
 // algorithm by Szabolcs Nagy, as built with fused multiply-adds on x86-64):
 // x = k ln2/128 + r, exp x = 2^(k/128) (1 + T_k + r + r^2 (C2 + r C3) + r^4 (C4 + r C5)).
+// The steps and the constants below are those of optimized-routines'
+// math/exp.c, Copyright (c) 2018 Arm Limited,
+// SPDX-License-Identifier: MIT OR Apache-2.0 WITH LLVM-exception; they are
+// reproduced because the result must round exactly as the host's libm does.
 BLDFM_HD double gl_exp(double x) {
   CR_EXACT
   const double ax = fabs(x);
@@ -361,7 +365,13 @@ BLDFM_HD double gl_exp(double x) {
 // 1 + x = 2^k (1 + f) with sqrt(2)/2 < 1 + f < sqrt(2), s = f/(2 + f),
 // log1p(f) = f - (f^2/2 - s (f^2/2 + R(s^2))), R a degree-7 polynomial in
 // s^2 evaluated in glibc's split order, plus k ln2 and the rounding
-// correction c/u of 1 + x.
+// correction c/u of 1 + x.  The steps and the constants are fdlibm's
+// (reproduced for the same reason as the exp above), under its notice:
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
 BLDFM_HD double gl_log1p(double x) {
   CR_EXACT
   const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
