@@ -69,3 +69,35 @@ def test_oracle_footprint_worker_unit():
     assert dt > 0
     np.testing.assert_array_equal(oc, fb.output(0)[0][0].numpy())
     np.testing.assert_array_equal(of, fb.output(0)[1][0].numpy())
+
+
+def test_traffic_only_for_the_profiled_kernel_sources(tmp_path, monkeypatch):
+    """roofline.traffic comes from the committed PMC summary only when the
+    summary's source_sha256 is the hash of the kernel sources bench.py runs."""
+    h = bench.kernel_source_hash()
+    assert h == bench.kernel_source_hash() and len(h) == 64
+    pmc = tmp_path / "pmc.txt"
+    monkeypatch.setattr(bench, "PMC_SUMMARY", str(pmc))
+    assert bench.measured_traffic()[0] is None  # no summary
+    pmc.write_text(f"source_sha256: {h}\ntraffic_bytes_per_launch k_shoot: 5.7e+08\n")
+    value, src = bench.measured_traffic()
+    assert value == 5.7e8 and "these kernel sources" in src
+    pmc.write_text(f"source_sha256: {'0' * 64}\ntraffic_bytes_per_launch k_shoot: 5.7e+08\n")
+    value, src = bench.measured_traffic()
+    assert value is None and "other kernel sources" in src
+
+
+def test_kernel_source_hash_follows_the_sources(tmp_path, monkeypatch):
+    """Changing any kernel source file changes the hash."""
+    import shutil
+
+    root = tmp_path / "r"
+    shutil.copytree(os.path.join(bench.ROOT, "bldfm_amd", "csrc"), root / "bldfm_amd" / "csrc")
+    (root / "include").mkdir()
+    shutil.copy(os.path.join(bench.ROOT, "include", "bldfm_amd.h"), root / "include")
+    monkeypatch.setattr(bench, "ROOT", str(root))
+    h0 = bench.kernel_source_hash()
+    assert h0 == bench.kernel_source_hash()
+    with open(root / "bldfm_amd" / "csrc" / "bldfm_march.h", "a") as f:
+        f.write("\n")
+    assert bench.kernel_source_hash() != h0
