@@ -462,7 +462,13 @@ int bldfm_plan_create_tuned(int device, const bldfm_instance* inst, int32_t nins
     D.vy = 1.0 / (D.nly * (1.0 / D.nly));
     D.shx = I.shift_x; D.shy = I.shift_y;
     D.p000 = I.srf_bg_conc;
-    D.q00 = 1.0 / D.nxe / D.nye;
+    // q^0 of the footprint's delta source: the reference divides a complex
+    // array by the two lengths (src/bldfm/solver.py:134), and numpy's complex
+    // division by a real-valued divisor is Smith's method, a product with the
+    // rounded reciprocal: (1/nxe) (1/nye), which differs from (1/nxe)/nye by
+    // one ulp for some lengths (960 x 384) -- and the shooting's cancellation
+    // turns one ulp of q^0 into 1e-8 of the weakest modes.
+    D.q00 = (1.0 / D.nxe) * (1.0 / D.nye);
 
     // levels: slot map then level index per slot
     D.off_slot = (int64_t)ints.size();

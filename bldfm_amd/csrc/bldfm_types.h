@@ -118,7 +118,7 @@ struct InstDesc {
   double vx, vy;        // fftfreq value factors 1/(nl*(1/nl))
   double shx, shy;      // phase-shift distances
   double p000;          // surface background concentration
-  double q00;           // footprint-mode delta spectrum (1/nxe)/nye
+  double q00;           // footprint-mode delta spectrum (1/nxe)(1/nye), rounded as numpy forms it
   // offsets
   int64_t off_prof;     // prof buffer: z,u,v,Kx,Ky,Kz (6*nz doubles)
   int64_t off_tab;      // level table: (nz-1)*BLDFM_LEVEL_STRIDE doubles

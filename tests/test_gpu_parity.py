@@ -351,6 +351,28 @@ def test_solver_length_without_split_is_refused(gpu):
                                              halo=8364.0, meas_pt=(1700.0, 8.0), footprint=True)
 
 
+@pytest.mark.parametrize("meas_pt,bar", [((-700.0, -700.0), 1e-10), ((50.0, 0.0), 1e-13)])
+def test_footprint_delta_spectrum_rounding(gpu, meas_pt, bar):
+    """The footprint's delta spectrum q^0 rounded as the reference forms it:
+    a complex array divided by nxe and nye (src/bldfm/solver.py:134), which
+    numpy evaluates with Smith's division as (1/nxe)(1/nye).  On the
+    fixture geometry (960 x 384 padded) that differs by one ulp from
+    (1/nxe)/nye, and this footprint is ill-conditioned in the shooting (the
+    oracle moves by 7e-9 (first case) and 5e-12 (second) when alpha moves by
+    one ulp).  With the tower at (-halo, -halo) the shift is exactly 1 and the
+    output depends on q^0 and the march alone.  With (1/nxe)/nye the device
+    was 3.2e-8 and 1.5e-12 off; oracle profiles for both."""
+    _, solver = gpu
+    z, profs = opbl.vertical_profiles(n=16, meas_height=10.0, wind=(0.0, -6.0), ustar=0.5)
+    q0 = np.zeros((128, 64))
+    kw = dict(modes=(64, 128), meas_pt=meas_pt, footprint=True, precision="double")
+    _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, (100.0, 700.0), 16, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, (100.0, 700.0), 16, **kw)
+    ec, ef = rel_l2(gc, oc), rel_l2(gf, of)
+    print(f"  meas_pt {meas_pt}: conc {ec:.2e} flx {ef:.2e} (bar {bar:g})")
+    assert ec <= bar and ef <= bar
+
+
 @pytest.mark.parametrize("nx,ny,modes,halo", [(129, 64, (64, 32), None), (64, 129, (32, 64), None),
                                               (129, 127, (64, 32), None), (129, 64, (64, 32), 0.0)])
 def test_odd_padded_length_footprint(gpu, nx, ny, modes, halo):

@@ -251,3 +251,17 @@ def test_odd_padded_length_footprint_follows_the_reference_sequence(nx, ny, mode
         assert ref.shape == out.shape and np.array_equal(ref, out)
     with pytest.raises(IndexError, match="boolean index did not match"):
         osolver.steady_state_transport_solver(np.ones((ny, nx)), z, profs, domain, 6, modes=modes, halo=halo)
+
+
+@pytest.mark.parametrize("nxe,nye", [(960, 384), (1536, 1536), (384, 384)])
+def test_footprint_delta_spectrum_follows_numpy_complex_division(nxe, nye):
+    """The reference forms the footprint's delta spectrum as
+    np.ones(..., complex128) / nxe / nye (src/bldfm/solver.py:134); numpy's
+    complex division by a real-valued divisor is Smith's method, a product
+    with the rounded reciprocal, so the value is (1/nxe)(1/nye) -- one ulp
+    away from (1/nxe)/nye at 960 x 384, the fixtures' padded grid.  The
+    library forms q^0 the same way."""
+    ref = (np.ones((2, 2), dtype=np.complex128) / nxe / nye)[0, 0]
+    assert ref.imag == 0.0 and ref.real == (1.0 / nxe) * (1.0 / nye)
+    if (nxe, nye) == (960, 384):
+        assert ref.real != 1.0 / nxe / nye
