@@ -1,0 +1,106 @@
+"""GPU: the drop-in chain on random geometries against the oracle chain.
+
+100 seeded random cases: grids of 12-96 cells per axis, even mode counts up
+to the grid, domains of 50-3000 m, default or random halos, footprint or
+dispersion (a random source, one to three random levels), MOST meteorology
+(u*, L of either sign, wind speed and direction, measurement height, with
+or without a domain height).  Each case runs device compute_wind_fields ->
+device vertical_profiles -> device solver, and the oracle chain from the
+same scalars.
+
+Bars: where the oracle raises, the device raises the same exception type
+(the reference's errors: odd padded-length parity in dispersion mode, empty
+vertical grids).  Otherwise rel-L2 of conc and flx <= max(1e-10, 8 s), with
+s the change of the oracle's own output when alpha moves by one ulp --
+dispersion levels above the measurement height can depend on the last bit
+of alpha (the growing and decaying solutions cancel), and no rounding-exact
+restatement can be held closer there than the reference is to itself on
+another CPU.  Every case's error and bar are printed.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import pbl as opbl
+from oracle import solver as osolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(seed, count):
+    rng = np.random.default_rng(seed)
+    for _ in range(count):
+        nx, ny = int(rng.integers(12, 97)), int(rng.integers(12, 97))
+        c = dict(nx=nx, ny=ny, modes=(2 * int(rng.integers(2, nx // 2 + 1)), 2 * int(rng.integers(2, ny // 2 + 1))),
+                 domain=(float(rng.uniform(50, 3000)), float(rng.uniform(50, 3000))))
+        c["halo"] = [None, float(rng.uniform(0, 1.5) * max(c["domain"]))][int(rng.integers(0, 2))]
+        c["footprint"] = bool(rng.integers(0, 2))
+        c["zm"], c["n"] = float(rng.uniform(2, 40)), int(rng.integers(6, 30))
+        c["ustar"], c["mol"] = float(rng.uniform(0.1, 1.0)), float(rng.choice([-1, 1]) * rng.uniform(5, 500))
+        c["speed"], c["dir"] = float(rng.uniform(1, 10)), float(rng.uniform(0, 360))
+        c["domain_height"] = [None, float(c["zm"] * rng.uniform(1.2, 6))][int(rng.integers(0, 2))]
+        c["level_draw"] = rng.integers(0, 10**6, size=int(rng.integers(1, 4))).tolist()
+        c["meas_pt"] = (float(rng.uniform(0, c["domain"][0])), float(rng.uniform(0, c["domain"][1])))
+        c["src_seed"] = int(rng.integers(0, 2**31))
+        yield c
+
+
+def _run(fn, *a, **k):
+    try:
+        return fn(*a, **k), None
+    except Exception as e:  # noqa: BLE001 -- the exception type is compared
+        return None, e
+
+
+@pytest.fixture(scope="module")
+def dropins():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from bldfm_amd import _lib, pbl_model, solver, utils
+
+    _lib.load()
+    return utils, pbl_model, solver
+
+
+def test_random_geometries_match_the_oracle_chain(dropins):
+    utils, pbl_model, solver = dropins
+    compared = raised = 0
+    worst = (0.0, None)
+    for i, c in enumerate(_cases(seed=20261203, count=100)):
+        u, v = utils.compute_wind_fields(c["speed"], c["dir"])
+        d = np.deg2rad(c["dir"])
+        ou, ov = c["speed"] * -np.sin(d), c["speed"] * -np.cos(d)  # src/bldfm/utils.py:23-27
+        kwp = dict(ustar=c["ustar"], mol=c["mol"], domain_height=c["domain_height"])
+        g, ge = _run(pbl_model.vertical_profiles, c["n"], c["zm"], (u, v), **kwp)
+        o, oe = _run(opbl.vertical_profiles, c["n"], c["zm"], (ou, ov), **kwp)
+        if oe is not None or ge is not None:
+            assert type(ge) is type(oe), (i, c, ge, oe)
+            raised += 1
+            continue
+        (z, prof), (oz, oprof) = g, o
+        nz = len(oz)
+        levels = c["n"] if c["footprint"] else sorted({k % nz for k in c["level_draw"]})
+        q0 = (np.zeros((c["ny"], c["nx"])) if c["footprint"]
+              else np.random.default_rng(c["src_seed"]).random((c["ny"], c["nx"])))
+        kw = dict(modes=c["modes"], meas_pt=c["meas_pt"], footprint=c["footprint"], halo=c["halo"],
+                  precision="double")
+        g, ge = _run(solver.steady_state_transport_solver, q0, z, prof, c["domain"], levels, **kw)
+        o, oe = _run(osolver.steady_state_transport_solver, q0, oz, oprof, c["domain"], levels, **kw)
+        if oe is not None or ge is not None:
+            assert type(ge) is type(oe), (i, c, ge, oe)
+            raised += 1
+            continue
+        (_, gc, gf), (_, oc, of) = g, o
+        _, ac, af = osolver.steady_state_transport_solver(q0, oz, oprof, c["domain"], levels, _alpha_ulps=1, **kw)
+        err = max(rel_l2(gc, oc), rel_l2(gf, of))
+        bar = max(1e-10, 8.0 * max(rel_l2(ac, oc), rel_l2(af, of)))
+        print(f"  case {i}: {c['nx']}x{c['ny']} modes {c['modes']} {'footprint' if c['footprint'] else 'dispersion'}"
+              f" levels {levels} of {nz}: rel-L2 {err:.2e} (bar {bar:.1e})", flush=True)
+        assert err <= bar, (i, c, err, bar)
+        compared += 1
+        if err > worst[0]:
+            worst = (err, i)
+    print(f"  {compared} compared, {raised} raised the reference's errors on both sides; worst {worst[0]:.2e} (case {worst[1]})")
+    assert compared >= 50
