@@ -46,6 +46,16 @@ def _cases(seed, count):
         yield c
 
 
+def _err(g, o):
+    """rel-L2 over the entries the oracle has finite; the device must have
+    NaN exactly where the oracle has (the reference's own grid can end in a
+    NaN level: a short stretch under a tall domain height)."""
+    g, o = np.asarray(g), np.asarray(o)
+    gn, on = np.isnan(g), np.isnan(o)
+    assert np.array_equal(gn, on), "NaN entries differ"
+    return 0.0 if on.all() else rel_l2(g[~on], o[~on])
+
+
 def _run(fn, *a, **k):
     try:
         return fn(*a, **k), None
@@ -104,3 +114,74 @@ def test_random_geometries_match_the_oracle_chain(dropins):
             worst = (err, i)
     print(f"  {compared} compared, {raised} raised the reference's errors on both sides; worst {worst[0]:.2e} (case {worst[1]})")
     assert compared >= 50
+
+
+def _option_cases(seed, count):
+    """Random closures and options: MOST/MOSTM/CONSTANT/OAAHOC, z0 instead of
+    u*, stretch, single precision, the analytic branch."""
+    rng = np.random.default_rng(seed)
+    for c in _cases(seed + 1, count):
+        c["closure"] = ["MOST", "MOSTM", "CONSTANT", "OAAHOC"][int(rng.integers(0, 4))]
+        c["prsc"] = float(rng.uniform(0.5, 1.5))
+        c["tke"] = float(rng.uniform(0.1, 3.0)) if c["closure"] == "OAAHOC" else None
+        c["z0_given"] = c["closure"] != "OAAHOC" and bool(rng.integers(0, 3) == 0)
+        c["z0"] = float(rng.uniform(0.01, 0.1) * c["zm"])
+        c["stretch"] = [None, float(c["zm"] * rng.uniform(0.5, 4))][int(rng.integers(0, 2))]
+        c["precision"] = ["double", "single"][int(rng.integers(0, 2))]
+        c["analytic"] = c["closure"] == "CONSTANT" and bool(rng.integers(0, 2))
+        yield c
+
+
+def test_random_options_match_the_oracle_chain(dropins):
+    """Bars as above; single precision: rel-L2 <= max(1e-6, 8 s) (float32
+    outputs; the unshifted single-precision transform differs from the
+    reference's float32 pyFFTW at that level)."""
+    utils, pbl_model, solver = dropins
+    compared = raised = 0
+    for i, c in enumerate(_option_cases(seed=20261204, count=100)):
+        u, v = utils.compute_wind_fields(c["speed"], c["dir"])
+        d = np.deg2rad(c["dir"])
+        ou, ov = c["speed"] * -np.sin(d), c["speed"] * -np.cos(d)
+        kwp = dict(mol=c["mol"], prsc=c["prsc"], closure=c["closure"], domain_height=c["domain_height"],
+                   stretch=c["stretch"], tke=c["tke"])
+        if c["z0_given"]:
+            kwp["z0"] = c["z0"]
+        else:
+            kwp["ustar"] = c["ustar"]
+        g, ge = _run(pbl_model.vertical_profiles, c["n"], c["zm"], (u, v), **kwp)
+        o, oe = _run(opbl.vertical_profiles, c["n"], c["zm"], (ou, ov), **kwp)
+        if oe is not None or ge is not None:
+            assert type(ge) is type(oe), (i, c, ge, oe)
+            raised += 1
+            continue
+        (z, prof), (oz, oprof) = g, o
+        nz = len(oz)
+        analytic = c["analytic"]
+        levels = c["n"] if (c["footprint"] or analytic) else sorted({k % nz for k in c["level_draw"]})
+        q0 = (np.zeros((c["ny"], c["nx"])) if c["footprint"]
+              else np.random.default_rng(c["src_seed"]).random((c["ny"], c["nx"])))
+        kw = dict(modes=c["modes"], meas_pt=c["meas_pt"], footprint=c["footprint"], halo=c["halo"],
+                  precision=c["precision"], analytic=analytic)
+        g, ge = _run(solver.steady_state_transport_solver, q0, z, prof, c["domain"], levels, **kw)
+        o, oe = _run(osolver.steady_state_transport_solver, q0, oz, oprof, c["domain"], levels, **kw)
+        if oe is not None or ge is not None:
+            assert type(ge) is type(oe), (i, c, ge, oe)
+            raised += 1
+            continue
+        (_, gc, gf), (_, oc, of) = g, o
+        assert gc.dtype == oc.dtype and gc.shape == oc.shape, (i, c, gc.dtype, oc.dtype, gc.shape, oc.shape)
+        for a, b in zip((z,) + tuple(prof), (oz,) + tuple(oprof)):
+            assert np.array_equal(np.isnan(a), np.isnan(b)), (i, c)
+        s = 0.0
+        if not analytic:
+            _, ac, af = osolver.steady_state_transport_solver(q0, oz, oprof, c["domain"], levels, _alpha_ulps=1, **kw)
+            s = max(_err(ac, oc), _err(af, of))
+        err = max(_err(gc, oc), _err(gf, of))
+        bar = max(1e-6 if c["precision"] == "single" else 1e-10, 8.0 * s)
+        print(f"  case {i}: {c['closure']}{' z0' if c['z0_given'] else ''}{' stretch' if c['stretch'] else ''} "
+              f"{c['precision']}{' analytic' if analytic else ''} {'footprint' if c['footprint'] else 'dispersion'} "
+              f"levels {levels} of {nz}: rel-L2 {err:.2e} (bar {bar:.1e})", flush=True)
+        assert err <= bar, (i, c, err, bar)
+        compared += 1
+    print(f"  {compared} compared, {raised} raised the reference's errors on both sides")
+    assert compared >= 40
