@@ -929,7 +929,16 @@ int bldfm_launch_fft2(const FftSplit& A0, int32_t n0, const FftSplit& A1, int32_
                       int threads, int lds_bytes, int grid_cap, hipStream_t stream) {
   const bool inv = sign > 0;
   const int64_t nrows = (int64_t)batch * n0, ncols = (int64_t)batch * n1;
-  const bool small = threads <= 256;
+  // kernel family by the longest LDS transform (a split axis runs its two
+  // factors): the 256-thread kernels hold 8 values per thread, so they take
+  // lengths up to BLDFM_FFT_SMALL_N; a longer transform may still run on 256
+  // threads (10 values each) in the other family
+  auto lds_n = [](const FftSplit& A, int32_t n) { return A.F1 ? (A.n1 > A.n2 ? A.n1 : A.n2) : n; };
+  const int nmax = lds_n(A0, n0) > lds_n(A1, n1) ? lds_n(A0, n0) : lds_n(A1, n1);
+  const bool small = nmax <= BLDFM_FFT_SMALL_N;
+  if ((small && threads > 256) || threads > 1024 ||
+      (int64_t)threads * (small ? FFT_SMALL_E : BLDFM_FFT_MAXE) < nmax)
+    return -1000 - (int)hipErrorInvalidConfiguration;
   // rows (axis 1, length n1): in -> mid
   if (A1.F1) {
     const unsigned ga = grid_for(nrows * A1.n2, grid_cap), gb = grid_for(nrows * A1.n1, grid_cap);

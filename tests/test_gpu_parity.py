@@ -351,6 +351,24 @@ def test_solver_length_without_split_is_refused(gpu):
                                              halo=8364.0, meas_pt=(1700.0, 8.0), footprint=True)
 
 
+@pytest.mark.parametrize("nx", [768, 853])
+def test_padded_lengths_between_the_kernel_families(gpu, nx):
+    """Padded lengths of 2049-2560 points (768 cells pad to 2304, 853 cells to
+    an odd 2559) run in the 10-values-per-thread FFT kernels on 256 threads,
+    the boundary case of the two kernel families; few modes keep the oracle
+    cheap.  FP64 parity with the oracle at rel-L2 <= 1e-10."""
+    _, solver = gpu
+    z, profs = opbl.vertical_profiles(n=12, meas_height=10.0, wind=(4.0, 1.0), ustar=0.4, mol=-60.0)
+    q0 = np.zeros((16, nx))
+    kw = dict(modes=(16, 16), meas_pt=(0.4 * nx, 0.5 * nx), footprint=True, precision="double")
+    dom = (float(nx), float(nx))  # 1 m cells in x, nx/16 m in y: padded nx x 3 by 48
+    _, gc, gf = solver.steady_state_transport_solver(q0, z, profs, dom, 12, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, profs, dom, 12, **kw)
+    ec, ef = rel_l2(gc, oc), rel_l2(gf, of)
+    print(f"  nx {nx}: conc {ec:.2e} flx {ef:.2e}")
+    assert gc.shape == oc.shape and ec <= 1e-10 and ef <= 1e-10
+
+
 @pytest.mark.parametrize("meas_pt,bar", [((-700.0, -700.0), 1e-10), ((50.0, 0.0), 1e-13)])
 def test_footprint_delta_spectrum_rounding(gpu, meas_pt, bar):
     """The footprint's delta spectrum q^0 rounded as the reference forms it:
