@@ -118,7 +118,8 @@ def test_random_geometries_match_the_oracle_chain(dropins):
 
 def _option_cases(seed, count):
     """Random closures and options: MOST/MOSTM/CONSTANT/OAAHOC, z0 instead of
-    u*, stretch, single precision, the analytic branch."""
+    u*, stretch, single precision, the analytic branch, a background
+    concentration, no halo, mode counts above the padded grid."""
     rng = np.random.default_rng(seed)
     for c in _cases(seed + 1, count):
         c["closure"] = ["MOST", "MOSTM", "CONSTANT", "OAAHOC"][int(rng.integers(0, 4))]
@@ -129,6 +130,12 @@ def _option_cases(seed, count):
         c["stretch"] = [None, float(c["zm"] * rng.uniform(0.5, 4))][int(rng.integers(0, 2))]
         c["precision"] = ["double", "single"][int(rng.integers(0, 2))]
         c["analytic"] = c["closure"] == "CONSTANT" and bool(rng.integers(0, 2))
+        c["srf_bg_conc"] = [0.0, float(rng.uniform(-2, 2))][int(rng.integers(0, 2))]
+        edge = int(rng.integers(0, 4))
+        if edge == 1:  # no halo: the padded grid is the grid
+            c["halo"] = 0.0
+        elif edge == 2:  # mode counts up to three times the grid: clamped to the padded size when above it
+            c["modes"] = (2 * int(rng.integers(2, 3 * c["nx"] // 2 + 1)), 2 * int(rng.integers(2, 3 * c["ny"] // 2 + 1)))
         yield c
 
 
@@ -161,7 +168,7 @@ def test_random_options_match_the_oracle_chain(dropins):
         q0 = (np.zeros((c["ny"], c["nx"])) if c["footprint"]
               else np.random.default_rng(c["src_seed"]).random((c["ny"], c["nx"])))
         kw = dict(modes=c["modes"], meas_pt=c["meas_pt"], footprint=c["footprint"], halo=c["halo"],
-                  precision=c["precision"], analytic=analytic)
+                  precision=c["precision"], analytic=analytic, srf_bg_conc=c["srf_bg_conc"])
         g, ge = _run(solver.steady_state_transport_solver, q0, z, prof, c["domain"], levels, **kw)
         o, oe = _run(osolver.steady_state_transport_solver, q0, oz, oprof, c["domain"], levels, **kw)
         if oe is not None or ge is not None:
