@@ -81,3 +81,47 @@ def test_bldfm_and_km01_neutral_comparison():
     ib_, ik_ = np.unravel_index(np.argmax(flx), flx.shape), np.unravel_index(np.argmax(ffm), ffm.shape)
     assert Y[ib_] > meas_pt[1] and ky[ik_] > meas_pt[1]
     assert np.hypot(X[ib_] - kx[ik_], Y[ib_] - ky[ik_]) < 50.0
+
+
+def test_random_cases_match_oracle():
+    """60 seeded random cases over the model's inputs (measurement height,
+    roughness, wind speed and direction or none, u*, Obukhov length of either
+    sign, sigma_v, grids and tower positions): the same grid, the same
+    warning and zero footprint where the model's validity checks reject the
+    input, and otherwise the same footprint at rel-L2 <= 1e-10.  The bar is
+    above the fixed cases' 1e-12: with a wind direction the grid is rotated
+    with the device's atan2/cos/sin (about one ulp, not correctly rounded),
+    and a narrow crosswind Gaussian (small sigma_v) turns one ulp of the
+    angle at 1 km into about 1e-12 of the footprint."""
+    from bldfm_amd.ffm_kormann_meixner import estimateFootprint
+
+    rng = np.random.default_rng(20261206)
+    compared = rejected = 0
+    worst = 0.0
+    for i in range(60):
+        x0 = float(rng.uniform(-600, 0)); y0 = float(rng.uniform(-600, 0))
+        c = dict(zm=float(rng.uniform(2, 40)), z0=float(np.exp(rng.uniform(np.log(0.01), np.log(2.0)))),
+                 ws=float(rng.uniform(0.5, 10)), ustar=float(rng.uniform(0.05, 1.0)),
+                 mo_len=float(rng.choice([-1, 1]) * np.exp(rng.uniform(np.log(5), np.log(1e5)))),
+                 sigma_v=float(rng.uniform(0.1, 1.5)),
+                 grid_domain=[x0, x0 + float(rng.uniform(200, 1500)), y0, y0 + float(rng.uniform(200, 1500))],
+                 grid_res=float(rng.uniform(2, 15)), mxy=[float(rng.uniform(-100, 100)), float(rng.uniform(-100, 100))],
+                 wd=[None, float(rng.uniform(0, 360))][int(rng.integers(0, 2))])
+        with warnings.catch_warnings(record=True) as wg:
+            warnings.simplefilter("always")
+            gx, gy, f = estimateFootprint(**c)
+        with warnings.catch_warnings(record=True) as wo:
+            warnings.simplefilter("always")
+            ox, oy, of = _oracle(c)
+        assert np.array_equal(gx, ox) and np.array_equal(gy, oy), (i, c)
+        assert len(wg) == len(wo), (i, c, [str(w.message) for w in wg], [str(w.message) for w in wo])
+        if not np.any(of):
+            assert not np.any(f), (i, c)
+            rejected += 1
+            continue
+        e = rel_l2(f, of)
+        assert e <= 1e-10 and np.all(f >= 0), (i, c, e)
+        compared += 1
+        worst = max(worst, e)
+    print(f"  {compared} compared, {rejected} rejected by both; worst rel-L2 {worst:.2e}")
+    assert compared >= 30
