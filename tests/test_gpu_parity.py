@@ -351,6 +351,28 @@ def test_solver_length_without_split_is_refused(gpu):
                                              halo=8364.0, meas_pt=(1700.0, 8.0), footprint=True)
 
 
+@pytest.mark.parametrize("levels", [-1, [2, -1], [-3, 0, 5]])
+@pytest.mark.parametrize("footprint", [False, True])
+def test_negative_levels(gpu, levels, footprint):
+    """Negative level indices: the reference's grid wraps them (z[levels])
+    while its snapshots test ``i in levels`` with i >= 0, so such a slot is
+    never filled by the march -- conc is zero there and flx keeps only the
+    conserved mean flux (src/bldfm/solver.py:191, 239-251, 345-372).  Same
+    shapes, grid and fields as the oracle at rel-L2 <= 1e-10."""
+    _, solver = gpu
+    z, profs = opbl.vertical_profiles(n=8, meas_height=10.0, wind=(3.0, 1.0), ustar=0.4, mol=-50.0)
+    q0 = np.zeros((16, 16)) if footprint else np.random.default_rng(3).random((16, 16))
+    kw = dict(modes=(8, 8), meas_pt=(40.0, 60.0), footprint=footprint, precision="double")
+    (gX, gY, gZ), gc, gf = solver.steady_state_transport_solver(q0, z, profs, (100.0, 100.0), levels, **kw)
+    (oX, oY, oZ), oc, of = osolver.steady_state_transport_solver(q0, z, profs, (100.0, 100.0), levels, **kw)
+    assert gc.shape == oc.shape and np.array_equal(gZ, oZ) and np.array_equal(gX, oX)
+    for a, b in ((gc, oc), (gf, of)):
+        if np.any(b):
+            assert rel_l2(a, b) <= 1e-10
+        else:
+            assert not np.any(a)
+
+
 @pytest.mark.parametrize("nx", [768, 853])
 def test_padded_lengths_between_the_kernel_families(gpu, nx):
     """Padded lengths of 2049-2560 points (768 cells pad to 2304, 853 cells to
