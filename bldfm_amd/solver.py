@@ -227,6 +227,8 @@ def steady_state_transport_solver(
     z = np.asarray(z, dtype=np.float64)
     xmx, ymx = domain
     lv = np.array([levels]) if np.ndim(levels) == 0 else levels
+    if len(lv) == 0:  # src/bldfm/solver.py:189 stores into slot 0 of the empty level axis
+        raise IndexError("index 0 is out of bounds for axis 0 with size 0")
     zl = z[lv]  # the reference indexes z by levels (IndexError for bad levels)
     conc_t, flx_t, g = solve_device(
         q0, z, profiles, domain, lv, modes, meas_pt, srf_bg_conc, footprint, analytic, halo, precision

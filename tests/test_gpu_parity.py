@@ -373,6 +373,35 @@ def test_negative_levels(gpu, levels, footprint):
             assert not np.any(a)
 
 
+@pytest.mark.parametrize("levels", [[2, 2], [5, 2], [3, 40], 40, [2.0, 5.0], np.array([1, 4]),
+                                    np.array([], dtype=int)], ids=str)
+def test_level_edge_cases(gpu, levels):
+    """Level lists as the reference takes them: duplicates (one slot filled,
+    the other left with the mean flux), unsorted lists (slots filled in
+    ascending level order, the grid in the caller's order), out-of-range,
+    non-integer and empty levels (the reference's IndexError).  The same
+    exception type, or the same shapes, grid and fields at rel-L2 <= 1e-10."""
+    _, solver = gpu
+    z, profs = opbl.vertical_profiles(n=8, meas_height=10.0, wind=(3.0, 1.0), ustar=0.4, mol=-50.0)
+    q0 = np.random.default_rng(5).random((16, 16))
+    kw = dict(modes=(8, 8), precision="double")
+    try:
+        og = osolver.steady_state_transport_solver(q0, z, profs, (100.0, 100.0), levels, **kw)
+    except Exception as e:  # noqa: BLE001 -- the type is compared
+        with pytest.raises(type(e)):
+            solver.steady_state_transport_solver(q0, z, profs, (100.0, 100.0), levels, **kw)
+        return
+    (gX, gY, gZ), gc, gf = solver.steady_state_transport_solver(q0, z, profs, (100.0, 100.0), levels, **kw)
+    (oX, oY, oZ), oc, of = og
+    assert gc.shape == oc.shape and np.array_equal(gZ, oZ)
+    for a, b in ((gc, oc), (gf, of)):
+        for k in range(b.shape[0]):
+            if np.any(b[k]):
+                assert rel_l2(a[k], b[k]) <= 1e-10
+            else:
+                assert not np.any(a[k])
+
+
 @pytest.mark.parametrize("nx", [768, 853])
 def test_padded_lengths_between_the_kernel_families(gpu, nx):
     """Padded lengths of 2049-2560 points (768 cells pad to 2304, 853 cells to
