@@ -157,6 +157,14 @@ def solve_device(
     shape = tuple(srf_flx.shape)
     z_np = z.detach().cpu().numpy() if torch.is_tensor(z) else np.asarray(z, dtype=np.float64)
     g = _geometry(shape, z_np, domain, levels, modes, meas_pt, footprint, halo, precision)
+    # the reference unpacks five arrays (src/bldfm/solver.py:84) and first
+    # indexes them at the top level nz-1 in this order (:164-173): shorter
+    # arrays raise its IndexError there, longer ones are used up to nz
+    u_, v_, kx_, ky_, kz_ = profiles
+    for a in (kz_, kx_, ky_, u_, v_):
+        if len(a) < g["nz"]:
+            raise IndexError(f"index {g['nz'] - 1} is out of bounds for axis 0 with size {len(a)}")
+    profiles = tuple(a[: g["nz"]] for a in (u_, v_, kx_, ky_, kz_))
     dev = _lib.device()
     nz, nslot = g["nz"], len(g["levels"])
     if nz < 2:

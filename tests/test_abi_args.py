@@ -66,3 +66,26 @@ def test_empty_batch_raises():
 
     with pytest.raises(ValueError, match="empty batch"):
         FootprintBatch([])
+
+
+@pytest.mark.parametrize("short", [0, 2, 4])
+def test_short_profile_arrays_raise_the_references_error(short):
+    """A profile array shorter than z raises the reference's IndexError at
+    its first top-level index (src/bldfm/solver.py:164-173: Kz, Kx, Ky, u, v
+    in that order), before any device work."""
+    import numpy as np
+
+    from bldfm_amd import solver
+    from oracle import pbl as opbl
+    from oracle import solver as osolver
+
+    z, profs = opbl.vertical_profiles(n=8, meas_height=10.0, wind=(3.0, 1.0), ustar=0.4, mol=-50.0)
+    profs = list(profs)
+    profs[short] = profs[short][:-2]
+    q0 = np.zeros((16, 16))
+    kw = dict(modes=(8, 8), meas_pt=(50.0, 50.0), footprint=True, precision="double")
+    with pytest.raises(IndexError) as oe:
+        osolver.steady_state_transport_solver(q0, z, tuple(profs), (100.0, 100.0), 8, **kw)
+    with pytest.raises(IndexError) as ge:
+        solver.steady_state_transport_solver(q0, z, tuple(profs), (100.0, 100.0), 8, **kw)
+    assert str(ge.value) == str(oe.value)

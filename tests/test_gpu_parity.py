@@ -402,6 +402,22 @@ def test_level_edge_cases(gpu, levels):
                 assert not np.any(a[k])
 
 
+def test_longer_profile_arrays_are_used_up_to_len_z(gpu):
+    """Profile arrays longer than z: the reference only indexes them below
+    len(z) (src/bldfm/solver.py:164-173, 336-368), so the extra entries are
+    ignored; the same result as the oracle and as the trimmed arrays."""
+    _, solver = gpu
+    z, profs = opbl.vertical_profiles(n=8, meas_height=10.0, wind=(3.0, 1.0), ustar=0.4, mol=-50.0)
+    longer = tuple(np.concatenate([p, np.full(5, 7.0)]) for p in profs)
+    q0 = np.zeros((16, 16))
+    kw = dict(modes=(8, 8), meas_pt=(50.0, 50.0), footprint=True, precision="double")
+    _, gc, gf = solver.steady_state_transport_solver(q0, z, longer, (100.0, 100.0), 8, **kw)
+    _, tc, tf = solver.steady_state_transport_solver(q0, z, profs, (100.0, 100.0), 8, **kw)
+    _, oc, of = osolver.steady_state_transport_solver(q0, z, longer, (100.0, 100.0), 8, **kw)
+    assert np.array_equal(gc, tc) and np.array_equal(gf, tf)
+    assert rel_l2(gc, oc) <= 1e-10 and rel_l2(gf, of) <= 1e-10
+
+
 @pytest.mark.parametrize("nx", [768, 853])
 def test_padded_lengths_between_the_kernel_families(gpu, nx):
     """Padded lengths of 2049-2560 points (768 cells pad to 2304, 853 cells to
