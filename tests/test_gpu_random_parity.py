@@ -192,3 +192,50 @@ def test_random_options_match_the_oracle_chain(dropins):
         compared += 1
     print(f"  {compared} compared, {raised} raised the reference's errors on both sides")
     assert compared >= 40
+
+
+def test_launch_shapes_are_bit_identical_on_random_geometries(dropins):
+    """Every launch shape of a plan (modes per workgroup of the march 64 /
+    128 / 256, x-pass input layout 0 / 1 / 2, the widest FFT workgroup) gives
+    the same bits on random geometries: odd-parity footprints, dispersion
+    with several levels, halos of every size."""
+    import torch
+
+    from bldfm_amd import _lib
+
+    _, _, solver = dropins
+    dev = _lib.device()
+    shapes = [dict(shoot_block=b, xpass_layout=x, fft_threads=0) for b in (64, 128, 256) for x in (0, 1, 2)]
+    checked = 0
+    for i, c in enumerate(_cases(seed=20261207, count=40)):
+        z, prof = opbl.vertical_profiles(c["n"], c["zm"], (3.0, -1.0), ustar=c["ustar"], mol=c["mol"])
+        nz = len(z)
+        levels = c["n"] if c["footprint"] else sorted({k % nz for k in c["level_draw"]})
+        try:
+            g = solver._geometry((c["ny"], c["nx"]), z, c["domain"], levels, c["modes"], c["meas_pt"],
+                                 c["footprint"], c["halo"], "double")
+        except Exception:  # noqa: BLE001 -- geometries the reference refuses
+            continue
+        slot, idx = solver._level_maps(g["levels"], nz)
+        ints = np.concatenate([slot, idx]).astype(np.int32)
+        inst = solver._instance(g, len(g["levels"]), c["footprint"], False, "double", 0.0)
+        pr = torch.cat([torch.as_tensor(np.asarray(a, dtype=np.float64)) for a in (z,) + tuple(prof)]).to(dev)
+        src = None
+        if not c["footprint"]:
+            src = torch.as_tensor(np.random.default_rng(c["src_seed"]).random((c["ny"], c["nx"]))).to(dev)
+        outs = []
+        for t in shapes + [dict(shoot_block=256, xpass_layout=0, fft_threads=1024)]:
+            try:
+                plan = _lib.Plan([inst], ints, dev, tuning=t)
+            except ValueError:  # the 256-thread FFT family: its widest workgroup
+                plan = _lib.Plan([inst], ints, dev, tuning=dict(t, fft_threads=256))
+            conc = torch.empty((len(g["levels"]), g["nyo"], g["nxo"]), dtype=torch.float64, device=dev)
+            flx = torch.empty_like(conc)
+            ws = torch.empty(max(plan.workspace_bytes, 1), dtype=torch.uint8, device=dev)
+            plan.execute(pr, src, conc, flx, ws)
+            outs.append((conc.cpu().numpy(), flx.cpu().numpy()))
+        for k, (a, b) in enumerate(outs[1:], 1):
+            assert np.array_equal(a, outs[0][0]) and np.array_equal(b, outs[0][1]), (i, c, k)
+        checked += 1
+    print(f"  {checked} geometries x {len(shapes) + 1} launch shapes bit-identical")
+    assert checked >= 20
