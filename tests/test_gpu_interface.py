@@ -110,3 +110,78 @@ def test_cli_run_writes_results(tmp_path):
         for i in range(2):
             assert np.array_equal(d[f"A/{i}/flx"], ref[i]["flx"])
             assert np.array_equal(d[f"A/{i}/X"], ref[i]["grid"][0])
+
+
+# ------------------------------------------------------------------ the reference's timeseries checks
+# Its tests/test_interface.py:151-250 on its conftest configuration
+# (tests/conftest.py:77-110): one tower, three met steps, default precision.
+
+def _timeseries_config():
+    from bldfm_amd.config_parser import parse_config_dict
+
+    return parse_config_dict({
+        "domain": {"nx": 64, "ny": 128, "xmax": 100.0, "ymax": 700.0, "nz": 16, "modes": [64, 128],
+                   "ref_lat": 50.95, "ref_lon": 11.586},
+        "towers": [{"name": "tower_A", "lat": 50.95, "lon": 11.5867, "z_m": 10.0}],
+        "met": {"ustar": [0.5, 0.4, 0.6], "mol": [-100.0, -200.0, -50.0], "wind_speed": [6.0, 5.0, 7.0],
+                "wind_dir": [0.0, 15.0, 345.0]},
+        "solver": {"closure": "MOST", "footprint": True},
+    })
+
+
+@pytest.fixture(scope="module")
+def timeseries():
+    from bldfm_amd import interface as itf
+
+    cfg = _timeseries_config()
+    return cfg, itf.run_bldfm_timeseries(cfg, cfg.towers[0])
+
+
+def test_timeseries_met_params_vary(timeseries):
+    cfg, results = timeseries
+    for r in results:
+        for key in ("ustar", "mol", "wind_speed", "wind_dir"):
+            assert key in r["params"]
+    for key in ("ustar", "mol", "wind_speed", "wind_dir"):
+        assert len({r["params"][key] for r in results}) > 1
+
+
+def test_timeseries_footprints_evolve_and_match_the_oracle(timeseries):
+    """The reference's shape/positivity/evolution checks, and each step
+    against the oracle chain from the same met values (the reference's
+    default precision "single": rel-L2 <= 1e-6)."""
+    cfg, results = timeseries
+    t = cfg.towers[0]
+    for i, r in enumerate(results):
+        assert r["flx"].shape == (cfg.domain.ny, cfg.domain.nx)
+        assert isinstance(r["grid"], tuple) and len(r["grid"]) == 3
+        assert r["flx"].max() > 0
+        step = cfg.met.get_step(i)
+        wd = np.deg2rad(step["wind_dir"])
+        wind = (-step["wind_speed"] * np.sin(wd), -step["wind_speed"] * np.cos(wd))
+        z, prof = opbl.vertical_profiles(16, t.z_m, wind, ustar=step["ustar"], mol=step["mol"])
+        _, oc, of = osolver.steady_state_transport_solver(
+            np.zeros((128, 64)), z, prof, (100.0, 700.0), 16, modes=(64, 128), meas_pt=(t.x, t.y), footprint=True)
+        assert r["flx"].dtype == of.dtype and rel_l2(r["flx"], of) <= 1e-6 and rel_l2(r["conc"], oc) <= 1e-6
+    assert any(not np.allclose(results[0]["flx"], results[i]["flx"]) for i in range(1, len(results)))
+
+
+def _percentile_contour(flx, grid, pct):
+    """Level and enclosed area of the smallest region holding pct of the
+    positive flux (the quantity abl-tk's extract_percentile_contour returns)."""
+    X, Y = grid[0], grid[1]
+    cell = abs(X[0, 1] - X[0, 0]) * abs(Y[1, 0] - Y[0, 0])
+    v = np.sort(flx[flx > 0].ravel())[::-1]
+    k = int(np.searchsorted(np.cumsum(v), pct * v.sum()))
+    return v[min(k, v.size - 1)], (k + 1) * cell
+
+
+def test_timeseries_aggregated_footprint(timeseries):
+    _, results = timeseries
+    grid = results[0]["grid"]
+    mean_flx = np.mean([r["flx"] for r in results], axis=0)
+    assert mean_flx.shape == results[0]["flx"].shape and np.isfinite(mean_flx).all()
+    level_50, area_50 = _percentile_contour(mean_flx, grid, 0.5)
+    level_70, area_70 = _percentile_contour(mean_flx, grid, 0.7)
+    assert level_50 > 0 and area_50 > 0 and level_70 > 0 and area_70 > 0
+    assert area_70 >= area_50 and level_50 >= level_70
