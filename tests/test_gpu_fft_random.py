@@ -54,3 +54,21 @@ def test_random_shapes_match_numpy():
             assert e <= 1e-11, (shape, norm, ours.__name__, e)
         print(f"  {shape} {norm}: ok", flush=True)
     print(f"  worst rel-L2 {worst:.2e}")
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.float32, np.float64, np.complex128])
+def test_output_dtype_follows_numpy(dtype):
+    """Output dtype as numpy gives it (single-precision inputs stay single),
+    values within the input precision's roundoff."""
+    from bldfm_amd import fft_manager
+
+    r = np.random.default_rng(3)
+    x = r.standard_normal((2, 48, 40))
+    if np.issubdtype(dtype, np.complexfloating):
+        x = x + 1j * r.standard_normal((2, 48, 40))
+    x = x.astype(dtype)
+    tol = 1e-6 if dtype in (np.complex64, np.float32) else 1e-13
+    for ours, ref in ((fft_manager.fft2, np.fft.fft2), (fft_manager.ifft2, np.fft.ifft2)):
+        a, b = ours(x), ref(x)
+        assert a.dtype == b.dtype, (dtype, a.dtype, b.dtype)
+        assert rel_l2(a, b) <= tol
