@@ -31,7 +31,6 @@ among its inputs.
 import ctypes
 import math
 from dataclasses import dataclass, field
-from typing import Optional
 
 import numpy as np
 import torch
