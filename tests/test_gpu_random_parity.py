@@ -239,3 +239,38 @@ def test_launch_shapes_are_bit_identical_on_random_geometries(dropins):
         checked += 1
     print(f"  {checked} geometries x {len(shapes) + 1} launch shapes bit-identical")
     assert checked >= 20
+
+
+def test_tiny_geometries(dropins):
+    """Grids of 1-8 cells per axis with 2-8 modes (odd padded lengths,
+    modes clamped to the padded grid, single-mode axes): the same results or
+    exceptions as the oracle; bars as in the first sweep."""
+    _, pbl_model, solver = dropins
+    rng = np.random.default_rng(20261208)
+    z, prof = opbl.vertical_profiles(10, 8.0, (2.0, 3.0), ustar=0.3, mol=-80.0)
+    compared = raised = 0
+    for i in range(80):
+        nx, ny = int(rng.integers(1, 9)), int(rng.integers(1, 9))
+        modes = (2 * int(rng.integers(1, 5)), 2 * int(rng.integers(1, 5)))
+        footprint = bool(rng.integers(0, 2))
+        dom = (float(rng.uniform(20, 400)), float(rng.uniform(20, 400)))
+        halo = [None, 0.0, float(rng.uniform(0, 2) * max(dom))][int(rng.integers(0, 3))]
+        levels = 10 if footprint else sorted({int(k) for k in rng.integers(0, len(z), 2)})
+        q0 = np.zeros((ny, nx)) if footprint else rng.random((ny, nx))
+        kw = dict(modes=modes, meas_pt=(0.3 * dom[0], 0.6 * dom[1]), footprint=footprint, halo=halo,
+                  precision="double")
+        g, ge = _run(solver.steady_state_transport_solver, q0, z, prof, dom, levels, **kw)
+        o, oe = _run(osolver.steady_state_transport_solver, q0, z, prof, dom, levels, **kw)
+        if oe is not None or ge is not None:
+            assert type(ge) is type(oe), (i, nx, ny, modes, footprint, halo, ge, oe)
+            raised += 1
+            continue
+        (_, gc, gf), (_, oc, of) = g, o
+        assert gc.shape == oc.shape, (i, gc.shape, oc.shape)
+        _, ac, af = osolver.steady_state_transport_solver(q0, z, prof, dom, levels, _alpha_ulps=1, **kw)
+        err = max(_err(gc, oc), _err(gf, of))
+        bar = max(1e-10, 8.0 * max(_err(ac, oc), _err(af, of)))
+        assert err <= bar, (i, nx, ny, modes, footprint, halo, err, bar)
+        compared += 1
+    print(f"  {compared} tiny geometries compared, {raised} raised the reference's errors on both sides")
+    assert compared >= 30
