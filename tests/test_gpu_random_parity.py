@@ -274,3 +274,60 @@ def test_tiny_geometries(dropins):
         compared += 1
     print(f"  {compared} tiny geometries compared, {raised} raised the reference's errors on both sides")
     assert compared >= 30
+
+
+def test_heterogeneous_plan_equals_single_instance_plans(dropins):
+    """One plan holding random dispersion and footprint instances of
+    different grids, padded lengths (several FFT plans in one launch chain)
+    and level counts: every instance's output equals its own single-instance
+    plan bit for bit."""
+    import torch
+
+    from bldfm_amd import _lib
+
+    _, _, solver = dropins
+    dev = _lib.device()
+    items = []
+    for c in _cases(seed=20261209, count=60):
+        if len(items) == 8:
+            break
+        z, prof = opbl.vertical_profiles(c["n"], c["zm"], (2.0, -1.5), ustar=c["ustar"], mol=c["mol"],
+                                         domain_height=c["domain_height"])
+        nz = len(z)
+        levels = c["n"] if c["footprint"] else sorted({k % nz for k in c["level_draw"]})
+        try:
+            g = solver._geometry((c["ny"], c["nx"]), z, c["domain"], levels, c["modes"], c["meas_pt"],
+                                 c["footprint"], c["halo"], "double")
+        except Exception:  # noqa: BLE001 -- geometries the reference refuses
+            continue
+        slot, idx = solver._level_maps(g["levels"], nz)
+        inst = solver._instance(g, len(g["levels"]), c["footprint"], False, "double", 0.0)
+        src = None if c["footprint"] else np.random.default_rng(c["src_seed"]).random((c["ny"], c["nx"]))
+        items.append((g, inst, np.concatenate([slot, idx]).astype(np.int32),
+                      np.concatenate([z] + [np.asarray(a, dtype=np.float64) for a in prof]), src))
+    assert len(items) == 8 and {it[1].footprint for it in items} == {0, 1}
+
+    def run(plan, parts):
+        prof = torch.zeros(max(plan.prof_doubles, 1), dtype=torch.float64)
+        src = torch.zeros(max(plan.src_doubles, 1), dtype=torch.float64)
+        for k, (_, _, _, p, s) in enumerate(parts):
+            op, osrc, _, _ = plan.offsets[k]
+            prof[op: op + p.size] = torch.from_numpy(p)
+            if s is not None:
+                src[osrc: osrc + s.size] = torch.from_numpy(s.ravel())
+        conc = torch.empty(max(plan.out_elems, 1), dtype=torch.float64, device=dev)
+        flx = torch.empty_like(conc)
+        ws = torch.empty(max(plan.workspace_bytes, 1), dtype=torch.uint8, device=dev)
+        plan.execute(prof.to(dev), src.to(dev) if plan.src_doubles > 0 else None, conc, flx, ws)
+        out = []
+        for k, (g, _, _, _, _) in enumerate(parts):
+            _, _, oo, _ = plan.offsets[k]
+            n = len(g["levels"]) * g["nyo"] * g["nxo"]
+            out.append((conc[oo: oo + n].cpu().numpy(), flx[oo: oo + n].cpu().numpy()))
+        return out
+
+    multi = run(_lib.Plan([it[1] for it in items], np.concatenate([it[2] for it in items]), dev), items)
+    for k, it in enumerate(items):
+        (c1, f1), = run(_lib.Plan([it[1]], it[2], dev), [it])
+        assert np.array_equal(multi[k][0], c1) and np.array_equal(multi[k][1], f1), k
+    print(f"  {len(items)} instances ({sum(it[1].footprint for it in items)} footprints) in one plan")
