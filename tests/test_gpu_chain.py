@@ -162,3 +162,37 @@ def test_device_oaahoc_tke_arrays(dropins):
         np.testing.assert_allclose(a, b, rtol=1e-14)
     with pytest.raises(ValueError, match="size 1"):
         pbl_model.vertical_profiles(10, 5.0, (3.0, 1.0), 0.4, closure="OAAHOC", tke=[2.0, 3.0])
+
+
+@pytest.mark.parametrize("kw", [dict(mol=np.inf), dict(mol=-np.inf), dict(mol=0.0), dict(mol=1e-300),
+                                dict(wind=(0.0, 0.0)), dict(ustar=1e-12), dict(mol=-1e-3), dict(mol=1e300)],
+                         ids=str)
+def test_device_profiles_on_degenerate_meteorology(dropins, kw):
+    """Neutral limits (L = +-inf), L = 0 (the reference's ZeroDivisionError),
+    L -> 0+, zero wind (NaN wind components) and u* -> 0 (infinite speeds):
+    the same exception, or the same values with NaN and inf in the same
+    places, as the oracle.  One deliberate difference: where the reference's
+    grid itself is not finite (L = 1e-300 makes the roughness length
+    infinite, so z[0] = inf), the drop-in raises ValueError instead of
+    returning NaN/inf profiles."""
+    _, pbl_model, _ = dropins
+    args = dict(n=8, meas_height=10.0, wind=(3.0, 1.0), ustar=0.4, mol=-50.0)
+    args.update(kw)
+    with np.errstate(all="ignore"):
+        try:
+            oz, op = opbl.vertical_profiles(**args)
+        except Exception as e:  # noqa: BLE001 -- the type is compared
+            with pytest.raises(type(e)):
+                pbl_model.vertical_profiles(**args)
+            return
+    if not np.all(np.isfinite(oz)):
+        with pytest.raises(ValueError, match="non-finite"):
+            pbl_model.vertical_profiles(**args)
+        return
+    z, p = pbl_model.vertical_profiles(**args)
+    for a, b in zip((z,) + tuple(p), (oz,) + tuple(op)):
+        assert a.shape == b.shape
+        assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(np.isinf(a), np.isinf(b))
+        fin = np.isfinite(b)
+        assert np.array_equal(np.sign(a[np.isinf(b)]), np.sign(b[np.isinf(b)]))
+        np.testing.assert_allclose(a[fin], b[fin], rtol=1e-12, atol=0)
