@@ -196,3 +196,18 @@ def test_device_profiles_on_degenerate_meteorology(dropins, kw):
         fin = np.isfinite(b)
         assert np.array_equal(np.sign(a[np.isinf(b)]), np.sign(b[np.isinf(b)]))
         np.testing.assert_allclose(a[fin], b[fin], rtol=1e-12, atol=0)
+
+
+def test_device_psi_phi_on_special_arguments(dropins):
+    """psi and phi (src/bldfm/pbl_model.py:207-250) at 0, -0, +-inf, NaN,
+    subnormal and huge arguments and across the unstable/stable switch: the
+    same values (bit for bit where finite), NaN and inf in the same places."""
+    _, pbl_model, _ = dropins
+    x = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, -5e-324, 1e-300, -1e-300, 1e300, -1e300,
+                  1e-8, -1e-8, 0.5, -0.5, 3.0, -3.0, 1e5, -1e5])
+    with np.errstate(all="ignore"):
+        for ours, ref in ((pbl_model.psi, opbl.psi), (pbl_model.phi, opbl.phi)):
+            a, b = np.asarray(ours(x), dtype=np.float64), np.asarray(ref(x), dtype=np.float64)
+            assert np.array_equal(np.isnan(a), np.isnan(b)), (ours.__name__, a, b)
+            fin = ~np.isnan(b)
+            assert np.array_equal(a[fin], b[fin]), (ours.__name__, x[fin][a[fin] != b[fin]])
